@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""SPC5 beta(r,8) SpMV benchmark on one MI355X.
+
+Workload (default): the clustered-runs matrix, N = 4,194,304 rows, lognormal
+row lengths (mean 32, sigma 0.8, clipped to [1,512]), runs of 1..16 columns
+within +-N/64 of the diagonal, values U[-1,1] (seed 1/2), x U[-1,1] (seed 3),
+double precision, converted on the GPU to beta(1|2|4|8, 8).
+
+A step is one y += A x over the whole matrix with every input resident in
+HBM.  W untimed steps, then K timed steps between two device
+synchronisations, timed with HIP events on the stream the kernel runs on.
+Prints one JSON line (rank 0).
+
+  value         GFLOP/s (2*nnz per step) of the fastest format
+  roofline      algorithmic bytes per launch / mean launch time vs 8 TB/s
+  cpu_baseline  the reference's OpenMP AVX-512 path (oracle/_ref) on the
+                host cores, on the same matrix (16 SpMVs, as its driver does)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "SpMV GFLOP/s + HBM GB/s (% of peak); mixed vs FP64 solve time at 1e-10"
+WORKLOADS = ("clustered", "laplacian", "stencil27")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name, n_override=None):
+    from spc5_amd import synth
+    if name == "clustered":
+        N = n_override or 4_194_304
+        rp, col, val = synth.clustered_runs(N, seed=1, value_seed=2)
+        desc = f"clustered_runs N={N} lognormal(mean 32, sigma 0.8) runs 1-16 within +-N/64, FP64"
+    elif name == "laplacian":
+        n = n_override or 512
+        rp, col, val = synth.laplacian2d(n)
+        desc = f"laplacian2d {n}x{n} (N={n * n}), FP64"
+    elif name == "stencil27":
+        n = n_override or 160
+        rp, col, val = synth.stencil27(n)
+        desc = f"stencil27 {n}^3 (N={n ** 3}), diag 26.001, FP64"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return rp, col, val, desc
+
+
+def time_spmv(m, x, y, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        m.spmv(x, y)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        m.spmv(x, y)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / steps, wall
+
+
+def copy_bandwidth():
+    """Device-to-device copy of 2 GiB (read + write counted)."""
+    import torch
+    n = (2 << 30) // 8
+    a = torch.empty(n, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 10
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * n * 8 / (ms * 1e-3) / 1e9
+
+
+def traffic_from_profiles(workload, fmt):
+    p = os.path.join(ROOT, "profiles", f"pmc_spmv_{workload}_{fmt}.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        return json.load(open(p)).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(rp, col, val, r, ncols, nthreads, nloops=16, max_rows=None):
+    """Reference OpenMP path (compiled reference header) on the host cores."""
+    import numpy as np
+    try:
+        import oracle as O
+    except Exception as e:  # noqa: BLE001
+        return {"error": f"oracle import failed: {e}"}
+    if not O.ref_available():
+        return {"error": "oracle/_ref/libspc5_ref.so not built"}
+    n = len(rp) - 1
+    if max_rows and max_rows < n:
+        n = max_rows
+        rp = rp[: n + 1]
+        col = col[: rp[-1]]
+        val = val[: rp[-1]]
+    nnz = int(rp[-1])
+    from spc5_amd import synth
+    x = np.zeros(ncols + 15)
+    x[:ncols] = synth.uniform(ncols, 3)
+    y = np.zeros(n + 7)
+    th, tconv, tspmv = O.ref_omp_bench(r, rp, col, val, x, y, nloops, nthreads, ncols=ncols)
+    gflops = 2.0 * nnz * nloops / tspmv / 1e9
+    return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": th, "kind": "reference",
+            "sample": f"{n} rows / {nnz} values of the same matrix, beta({r},8) FP64, "
+                      f"SPC5_Spmv_omp x{nloops} after split (conversion {tconv:.1f}s untimed)",
+            "ms_per_spmv": round(tspmv / nloops * 1e3, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="clustered", choices=WORKLOADS)
+    ap.add_argument("--n", type=int, default=None, help="override the workload size")
+    ap.add_argument("--formats", default="1,2,4,8")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-max-rows", type=int, default=0)
+    ap.add_argument("--no-copy", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the SpMV would shard by "
+              "block-row ranges with an x all-gather; see the design notes). Run with --gpus 1.",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    from spc5_amd import synth
+    from spc5_amd.device import DeviceMatrix
+
+    assert torch.cuda.is_available(), "needs a HIP device"
+    t_all = time.perf_counter()
+    rp, col, val, desc = make_workload(args.workload, args.n)
+    n = len(rp) - 1
+    nnz = int(rp[-1])
+    ncols = n
+    log(f"[bench] workload {desc}: nnz={nnz} ({time.perf_counter() - t_all:.1f}s)")
+    d_rp = torch.from_numpy(rp).cuda()
+    d_col = torch.from_numpy(col).cuda()
+    d_val = torch.from_numpy(val).cuda()
+    x = torch.from_numpy(synth.uniform(ncols, 3)).cuda()
+    y = torch.zeros(n, dtype=torch.float64, device="cuda")
+
+    formats = {}
+    best = None
+    for r in [int(s) for s in args.formats.split(",")]:
+        tc = time.perf_counter()
+        m = DeviceMatrix.from_csr(d_rp, d_col, d_val, r)
+        torch.cuda.synchronize()
+        conv_s = time.perf_counter() - tc
+        ms, wall = time_spmv(m, x, y, args.steps, args.warmup)
+        B = m.algorithmic_bytes()
+        gbs = B / (ms * 1e-3) / 1e9
+        gflops = 2.0 * nnz / (ms * 1e-3) / 1e9
+        info = m.info()
+        formats[m.format_name] = {"r": r, "nblocks": m.nblocks, "ntiles": info["ntiles"],
+                                  "bytes_per_spmv": B, "ms": round(ms, 5), "gflops": round(gflops, 2),
+                                  "gbs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+                                  "convert_s": round(conv_s, 3), "timed_wall_s": round(wall, 3)}
+        log(f"[bench] beta({r},8): {ms:.4f} ms  {gflops:.1f} GFLOP/s  {gbs:.0f} GB/s  "
+            f"({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)  nb={m.nblocks} tiles={info['ntiles']}")
+        if best is None or ms < formats[best]["ms"]:
+            best = m.format_name
+        del m
+        torch.cuda.empty_cache()
+
+    bf = formats[best]
+    copy_gbs = None if args.no_copy else round(copy_bandwidth(), 1)
+    out = {
+        "metric": METRIC,
+        "value": bf["gflops"],
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": bf["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "nrows": n, "ncols": ncols, "nnz": nnz, "format": f"beta({bf['r']},8)",
+                   "step": "y += A x, inputs resident in HBM"},
+        "roofline": {"bound": "hbm", "achieved": bf["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": bf["frac_of_peak"],
+                     "traffic": traffic_from_profiles(args.workload, best)},
+        "formats": formats,
+        "copy_gbs_measured": copy_gbs,
+    }
+    if not args.no_cpu:
+        log("[bench] CPU baseline (reference OpenMP path)...")
+        th = args.cpu_threads or os.cpu_count() or 1
+        out["cpu_baseline"] = cpu_baseline(rp, col, val, bf["r"], ncols, th,
+                                           max_rows=args.cpu_max_rows or None)
+    else:
+        out["cpu_baseline"] = None
+    log(f"[bench] total {time.perf_counter() - t_all:.1f}s")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

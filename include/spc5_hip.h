@@ -1,0 +1,159 @@
+/*
+ * spc5_hip.h -- C ABI of the MI355X (gfx950) SPC5 library, libspc5_hip.so.
+ *
+ * Two groups of entry points:
+ *
+ *  1. The reference's own C ABI, same names, same arguments, same
+ *     semantics (y += A*x on host pointers).  A program built against the
+ *     reference header links against this library instead and its SpMV
+ *     calls run on the GPU.  These copy their inputs to the GPU, run the
+ *     HIP kernel and copy y back; they are the drop-in, not the fast path.
+ *
+ *  2. spc5_hip_* -- the device-resident API: caller-owned device buffers,
+ *     explicit sizes, an explicit stream (a hipStream_t passed as void*;
+ *     NULL = the default stream), an int status code (SPC5_HIP_OK = 0), and
+ *     a message from spc5_hip_last_error() on failure.  Nothing here throws.
+ *
+ * Matrix layout (identical to the reference, src/spc5.hpp:44-62, 165-228):
+ *   rowsSize[ceil(N/r)+1]   int32 block-row pointers (first block of each
+ *                           block-row; the last entry is the block count)
+ *   headers[nb*(4+r*MB)]    per block: int32 first column (little endian)
+ *                           then r masks of MB bytes; bit k of mask s set
+ *                           <=> entry (row b*r+s, column start+k) is stored
+ *   values[nnz]             stored entries, block after block, and inside a
+ *                           block row after row, columns ascending
+ *   double: VS=8 columns, MB=1; float: VS=16, MB=2 (the reference formats).
+ *   Extension formats of this library: float or bfloat16 values with VS=8,
+ *   MB=1 (the double-precision block structure with cheaper values).
+ */
+#ifndef SPC5_HIP_H
+#define SPC5_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Status codes                                                           */
+/* ---------------------------------------------------------------------- */
+#define SPC5_HIP_OK 0
+#define SPC5_HIP_ERR_INVALID 1     /* bad argument or unsupported combination */
+#define SPC5_HIP_ERR_HIP 2         /* a HIP runtime call failed */
+#define SPC5_HIP_ERR_ALLOC 3       /* device allocation failed */
+#define SPC5_HIP_ERR_OVERFLOW 4    /* a count does not fit the 32-bit format */
+
+/* Value types */
+#define SPC5_HIP_F64 0   /* values, x, y and sums in double */
+#define SPC5_HIP_F32 1   /* values, x, y and sums in float */
+#define SPC5_HIP_BF16 2  /* bfloat16 values (raw uint16), float x, y and sums */
+
+/* ---------------------------------------------------------------------- */
+/* 1. Reference C ABI (drop-in).                                          */
+/* ---------------------------------------------------------------------- */
+
+/* y[i] += (A x)[i] for the beta(r,VS) matrix given by the reference arrays.
+ * nbRows is the number of scalar rows; rowsSizes is indexed by block-row and
+ * only differences are used; headers and values are read from their first
+ * element.  x must hold every column any block touches (the reference pads
+ * it by VS-1); rows past nbRows are not written.
+ * Replaces src/spc5.hpp:516-523 (1rVc), 641-648 (2rVc), 769-776 (4rVc),
+ * 895-902 (8rVc).  Aborts with a message on a GPU error (the reference
+ * interface has no error channel). */
+void core_SPC5_1rVc_Spmv_double(const long int nbRows, const int* rowsSizes,
+                                const unsigned char* blocksColumnIndexesWithMasks,
+                                const double* values, const double* x, double* y);
+void core_SPC5_1rVc_Spmv_float(const long int nbRows, const int* rowsSizes,
+                               const unsigned char* blocksColumnIndexesWithMasks,
+                               const float* values, const float* x, float* y);
+void core_SPC5_2rVc_Spmv_double(const long int nbRows, const int* rowsSizes,
+                                const unsigned char* blocksColumnIndexesWithMasks,
+                                const double* values, const double* x, double* y);
+void core_SPC5_2rVc_Spmv_float(const long int nbRows, const int* rowsSizes,
+                               const unsigned char* blocksColumnIndexesWithMasks,
+                               const float* values, const float* x, float* y);
+void core_SPC5_4rVc_Spmv_double(const long int nbRows, const int* rowsSizes,
+                                const unsigned char* blocksColumnIndexesWithMasks,
+                                const double* values, const double* x, double* y);
+void core_SPC5_4rVc_Spmv_float(const long int nbRows, const int* rowsSizes,
+                               const unsigned char* blocksColumnIndexesWithMasks,
+                               const float* values, const float* x, float* y);
+void core_SPC5_8rVc_Spmv_double(const long int nbRows, const int* rowsSizes,
+                                const unsigned char* blocksColumnIndexesWithMasks,
+                                const double* values, const double* x, double* y);
+void core_SPC5_8rVc_Spmv_float(const long int nbRows, const int* rowsSizes,
+                               const unsigned char* blocksColumnIndexesWithMasks,
+                               const float* values, const float* x, float* y);
+
+/* dest[i] += src[i] for i < nbValues (host pointers).
+ * Replaces src/spc5.hpp:434-435 (declared), 2157-2175 (AVX-512 body). */
+void SPC5_opti_merge_double(double dest[], const double src[], const int nbValues);
+void SPC5_opti_merge_float(float dest[], const float src[], const int nbValues);
+
+/* ---------------------------------------------------------------------- */
+/* 2. Device-resident API                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* Message of the last failing call in this process ("" if none). */
+const char* spc5_hip_last_error(void);
+
+/* Library version string. */
+const char* spc5_hip_version(void);
+
+typedef struct spc5_hip_mat* spc5_hip_mat_t;
+
+/* Wrap device arrays in the reference layout (above) into a matrix handle
+ * and build its work plan on the GPU (block-row value offsets and the
+ * nnz-balanced tile list).  The arrays stay caller-owned and must outlive
+ * the handle.  (value_type, vs, mask bytes) must be one of
+ *   (F64, 8, 1), (F32, 16, 2), (F32, 8, 1), (BF16, 8, 1).
+ * Counterpart of holding an SPC5Mat after CSR_to_SPC5 (src/spc5.hpp:44-62,
+ * 989-1018) plus the work split of core_SPC5_rVc_threadsplit
+ * (src/spc5.hpp:346-427), which the tile plan replaces. */
+int spc5_hip_mat_create(int value_type, int vs, int r, int64_t nrows, int64_t ncols,
+                        int64_t nnz, int64_t nblocks, const int* d_rowsSize,
+                        const unsigned char* d_headers, const void* d_values,
+                        void* stream, spc5_hip_mat_t* out);
+
+/* Same, for a matrix whose block structure is shared with another handle
+ * (same r, VS, rowsSize and headers) but with other values: used for the
+ * float/bfloat16 copies of a double matrix.  Reuses the plan of `base`. */
+int spc5_hip_mat_create_values_view(spc5_hip_mat_t base, int value_type, const void* d_values,
+                                    spc5_hip_mat_t* out);
+
+int spc5_hip_mat_destroy(spc5_hip_mat_t h);
+
+/* Plan and shape information: number of tiles, block-rows, blocks, nnz. */
+int spc5_hip_mat_info(spc5_hip_mat_t h, int64_t* ntiles, int64_t* nblockrows, int64_t* nblocks,
+                      int64_t* nnz);
+
+/* y += A x on device pointers (x and y in the matrix's vector type).
+ * Replaces core_SPC5_<r>rVc_Spmv_<T> (src/spc5.hpp:2309-2715) and the
+ * OpenMP driver SPC5_<r>rVc_Spmv_omp (src/spc5.hpp:557-599 etc.). */
+int spc5_hip_spmv(spc5_hip_mat_t h, const void* d_x, void* d_y, void* stream);
+
+/* GPU converter, CSR (device) -> beta(r,VS), in two calls.
+ * count: writes the block-row pointers into d_rowsSize_out[0..ceil(N/r)]
+ *   (count pass + exclusive scan) and returns the block count in
+ *   *nblocks (synchronises the stream to read it).
+ * fill : writes the packed headers (nblocks*(4+r*MB) bytes) and the values
+ *   permuted into block order (nnz * value_bytes).
+ * d_rowsSize_out must not alias d_rowptr.  Columns must be sorted and
+ * unique inside each row, as the reference requires.
+ * Replaces CSR_to_SPC5_<r>rVc / CSR_to_SPC5 (src/spc5.hpp:165-228,
+ * 455-488, 609-613, 738-742, 864-868, 989-1018) and the block counters
+ * SPC5_<r>rVc_block_count (src/spc5.hpp:276-317, 510-513, 635-638,
+ * 764-767, 890-893). */
+int spc5_hip_convert_count(int vs, int r, int64_t nrows, const int* d_rowptr, const int* d_colidx,
+                           int* d_rowsSize_out, int64_t* nblocks, void* stream);
+int spc5_hip_convert_fill(int value_bytes, int vs, int mask_bytes, int r, int64_t nrows,
+                          const int* d_rowptr, const int* d_colidx, const void* d_values_in,
+                          const int* d_rowsSize, unsigned char* d_headers_out, void* d_values_out,
+                          void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPC5_HIP_H */

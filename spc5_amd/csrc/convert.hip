@@ -1,0 +1,264 @@
+// convert.hip -- CSR -> beta(r,VS) on the GPU, byte-identical to the
+// reference converter (src/spc5.hpp:165-228 and 455-488).
+//
+// The reference walks each block-row with one cursor per row: a block starts
+// at the smallest unconsumed column of the block-row and takes, row by row,
+// every entry with column < start+VS.  Block-rows are independent, so the
+// GPU runs one lane per block-row in two passes:
+//   count : blocks per block-row -> exclusive scan -> block-row pointers
+//           (the reference's rowsSize after conversion) and the block count;
+//   fill  : the same walk again, writing each block's packed header at
+//           (pointer + k) * (4 + r*MB) bytes and its values at the block-row's
+//           value offset, which is the CSR row pointer of its first row
+//           (conversion only permutes values inside a block-row).
+#include "spc5_common.hpp"
+
+#include <climits>
+
+namespace spc5 {
+
+namespace {
+
+// Checks a CSR row pointer: starts at 0, non-decreasing.
+__global__ void k_check_rowptr(const int* __restrict__ rowptr, int64_t nrows, int* err) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nrows) return;
+    bool bad = false;
+    if (i == 0) bad = rowptr[0] != 0;
+    if (i < nrows) bad |= rowptr[i + 1] < rowptr[i];
+    if (bad) atomicOr(err, 1);
+}
+
+template <int R>
+struct Cursors {
+    int cur[R];
+    int end[R];
+    int head[R];  // column at the cursor, INT_MAX when the row is consumed
+
+    __device__ __forceinline__ void init(int64_t row0, int64_t nrows, const int* rowptr, const int* col) {
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            if (row0 + s < nrows) {
+                cur[s] = rowptr[row0 + s];
+                end[s] = rowptr[row0 + s + 1];
+            } else {
+                cur[s] = end[s] = 0;
+            }
+            head[s] = cur[s] < end[s] ? col[cur[s]] : INT_MAX;
+        }
+    }
+    // Smallest unconsumed column; false when every row is consumed.
+    __device__ __forceinline__ bool next_start(int& start) const {
+        bool any = false;
+        start = INT_MAX;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            if (cur[s] < end[s]) {
+                any = true;
+                start = head[s] < start ? head[s] : start;
+            }
+        }
+        return any;
+    }
+};
+
+template <int R, int VS>
+__global__ void k_convert_count(int64_t nrows, int64_t nbr, const int* __restrict__ rowptr,
+                                const int* __restrict__ col, int* __restrict__ cnt_out) {
+    const int64_t br = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (br >= nbr) return;
+    Cursors<R> c;
+    c.init(br * R, nrows, rowptr, col);
+    int nb = 0;
+    int start;
+    while (c.next_start(start)) {
+        ++nb;
+        const int64_t lim = (int64_t)start + VS;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            while (c.cur[s] < c.end[s] && c.head[s] < lim) {
+                ++c.cur[s];
+                c.head[s] = c.cur[s] < c.end[s] ? col[c.cur[s]] : INT_MAX;
+            }
+        }
+    }
+    cnt_out[br] = nb;
+}
+
+template <class W, int R, int VS, int MB>
+__global__ void k_convert_fill(int64_t nrows, int64_t nbr, const int* __restrict__ rowptr,
+                               const int* __restrict__ col, const W* __restrict__ vin,
+                               const int* __restrict__ bptr, int64_t nblocks, unsigned char* __restrict__ hdr,
+                               W* __restrict__ vout) {
+    constexpr int H = 4 + R * MB;
+    constexpr int NW = (R * MB + 7) / 8;  // 64-bit words of the combined mask
+    const int64_t br = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (br >= nbr) return;
+    const int64_t row0 = br * R;
+    Cursors<R> c;
+    c.init(row0, nrows, rowptr, col);
+    int64_t bo = bptr[br];
+    const int64_t bend = bptr[br + 1] < nblocks ? bptr[br + 1] : nblocks;
+    int64_t vo = rowptr[row0];
+    int start;
+    while (c.next_start(start)) {
+        if (bo >= bend) break;  // inconsistent pointers: never write past the block-row
+        const int64_t lim = (int64_t)start + VS;
+        unsigned long long mw[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) mw[k] = 0ull;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            while (c.cur[s] < c.end[s] && c.head[s] < lim) {
+                const int bit = s * VS + (c.head[s] - start);
+                mw[bit >> 6] |= 1ull << (bit & 63);
+                vout[vo++] = vin[c.cur[s]];
+                ++c.cur[s];
+                c.head[s] = c.cur[s] < c.end[s] ? col[c.cur[s]] : INT_MAX;
+            }
+        }
+        unsigned char* h = hdr + bo * H;
+        __builtin_memcpy(h, &start, 4);
+        __builtin_memcpy(h + 4, mw, R * MB);
+        ++bo;
+    }
+}
+
+template <int R, int VS>
+void launch_count(int64_t nrows, int64_t nbr, const int* rowptr, const int* col, int* out, hipStream_t s) {
+    const unsigned g = (unsigned)((nbr + 255) / 256);
+    hipLaunchKernelGGL((k_convert_count<R, VS>), dim3(g), dim3(256), 0, s, nrows, nbr, rowptr, col, out);
+}
+
+template <class W, int VS, int MB>
+int launch_fill_r(int r, int64_t nrows, int64_t nbr, const int* rowptr, const int* col, const void* vin,
+                  const int* bptr, int64_t nblocks, unsigned char* hdr, void* vout, hipStream_t s) {
+    const unsigned g = (unsigned)((nbr + 255) / 256);
+    const W* vi = reinterpret_cast<const W*>(vin);
+    W* vo = reinterpret_cast<W*>(vout);
+    switch (r) {
+    case 1:
+        hipLaunchKernelGGL((k_convert_fill<W, 1, VS, MB>), dim3(g), dim3(256), 0, s, nrows, nbr, rowptr, col, vi, bptr,
+                           nblocks, hdr, vo);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_convert_fill<W, 2, VS, MB>), dim3(g), dim3(256), 0, s, nrows, nbr, rowptr, col, vi, bptr,
+                           nblocks, hdr, vo);
+        break;
+    case 4:
+        hipLaunchKernelGGL((k_convert_fill<W, 4, VS, MB>), dim3(g), dim3(256), 0, s, nrows, nbr, rowptr, col, vi, bptr,
+                           nblocks, hdr, vo);
+        break;
+    case 8:
+        hipLaunchKernelGGL((k_convert_fill<W, 8, VS, MB>), dim3(g), dim3(256), 0, s, nrows, nbr, rowptr, col, vi, bptr,
+                           nblocks, hdr, vo);
+        break;
+    default: set_error("r must be 1, 2, 4 or 8"); return SPC5_HIP_ERR_INVALID;
+    }
+    SPC5_HIP_TRY(hipGetLastError());
+    return SPC5_HIP_OK;
+}
+
+template <class W>
+int launch_fill_w(int vs, int mb, int r, int64_t nrows, int64_t nbr, const int* rowptr, const int* col,
+                  const void* vin, const int* bptr, int64_t nblocks, unsigned char* hdr, void* vout, hipStream_t s) {
+    if (vs == 8 && mb == 1) return launch_fill_r<W, 8, 1>(r, nrows, nbr, rowptr, col, vin, bptr, nblocks, hdr, vout, s);
+    if (vs == 16 && mb == 2)
+        return launch_fill_r<W, 16, 2>(r, nrows, nbr, rowptr, col, vin, bptr, nblocks, hdr, vout, s);
+    set_error("(VS, mask bytes) must be (8, 1) or (16, 2)");
+    return SPC5_HIP_ERR_INVALID;
+}
+
+int check_rowptr(const int* rowptr, int64_t nrows, hipStream_t s) {
+    DeviceBuffer err;
+    SPC5_HIP_TRY(err.alloc(sizeof(int)));
+    SPC5_HIP_TRY(hipMemsetAsync(err.p, 0, sizeof(int), s));
+    const unsigned g = (unsigned)((nrows + 1 + 255) / 256);
+    hipLaunchKernelGGL(k_check_rowptr, dim3(g), dim3(256), 0, s, rowptr, nrows, err.as<int>());
+    SPC5_HIP_TRY(hipGetLastError());
+    int h = 0;
+    SPC5_HIP_TRY(hipMemcpyAsync(&h, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPC5_HIP_TRY(hipStreamSynchronize(s));
+    if (h) {
+        set_error("CSR row pointer must start at 0 and be non-decreasing");
+        return SPC5_HIP_ERR_INVALID;
+    }
+    return SPC5_HIP_OK;
+}
+
+}  // namespace
+
+}  // namespace spc5
+
+using namespace spc5;
+
+extern "C" {
+
+int spc5_hip_convert_count(int vs, int r, int64_t nrows, const int* d_rowptr, const int* d_colidx,
+                           int* d_rowsSize_out, int64_t* nblocks, void* stream) {
+    SPC5_CHECK_ARG(nblocks != nullptr, "nblocks is null");
+    SPC5_CHECK_ARG(r == 1 || r == 2 || r == 4 || r == 8, "r must be 1, 2, 4 or 8");
+    SPC5_CHECK_ARG(vs == 8 || vs == 16, "VS must be 8 or 16");
+    SPC5_CHECK_ARG(nrows >= 0 && nrows <= INT32_MAX, "nrows out of range");
+    SPC5_CHECK_ARG(d_rowptr != nullptr && d_rowsSize_out != nullptr, "null pointer");
+    SPC5_CHECK_ARG((const void*)d_rowptr != (const void*)d_rowsSize_out, "rowsSize_out must not alias rowptr");
+    hipStream_t s = as_stream(stream);
+    *nblocks = 0;
+    const int64_t nbr = (nrows + r - 1) / r;
+    int rc = check_rowptr(d_rowptr, nrows, s);
+    if (rc) return rc;
+    SPC5_HIP_TRY(hipMemsetAsync(d_rowsSize_out + nbr, 0, sizeof(int), s));
+    if (nbr > 0) {
+        const int key = r * 100 + vs;
+        switch (key) {
+        case 108: launch_count<1, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
+        case 208: launch_count<2, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
+        case 408: launch_count<4, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
+        case 808: launch_count<8, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
+        case 116: launch_count<1, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
+        case 216: launch_count<2, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
+        case 416: launch_count<4, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
+        case 816: launch_count<8, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
+        }
+        SPC5_HIP_TRY(hipGetLastError());
+    }
+    DeviceBuffer total;
+    SPC5_HIP_TRY(total.alloc(sizeof(int)));
+    rc = scan_exclusive_i32(d_rowsSize_out, nbr + 1, total.as<int>(), s);
+    if (rc) return rc;
+    int h = 0;
+    SPC5_HIP_TRY(hipMemcpyAsync(&h, total.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPC5_HIP_TRY(hipStreamSynchronize(s));
+    *nblocks = h;
+    return SPC5_HIP_OK;
+}
+
+int spc5_hip_convert_fill(int value_bytes, int vs, int mask_bytes, int r, int64_t nrows, const int* d_rowptr,
+                          const int* d_colidx, const void* d_values_in, const int* d_rowsSize,
+                          unsigned char* d_headers_out, void* d_values_out, void* stream) {
+    SPC5_CHECK_ARG(value_bytes == 8 || value_bytes == 4 || value_bytes == 2, "value_bytes must be 8, 4 or 2");
+    SPC5_CHECK_ARG(nrows >= 0 && nrows <= INT32_MAX, "nrows out of range");
+    SPC5_CHECK_ARG(d_rowptr != nullptr && d_rowsSize != nullptr, "null pointer");
+    hipStream_t s = as_stream(stream);
+    const int64_t nbr = (nrows + r - 1) / r;
+    if (nbr == 0) return SPC5_HIP_OK;
+    int nb = 0;
+    SPC5_HIP_TRY(hipMemcpyAsync(&nb, d_rowsSize + nbr, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPC5_HIP_TRY(hipStreamSynchronize(s));
+    if (nb == 0) return SPC5_HIP_OK;
+    SPC5_CHECK_ARG(d_headers_out != nullptr && d_values_out != nullptr && d_values_in != nullptr,
+                   "null output/value pointer");
+    switch (value_bytes) {
+    case 8:
+        return launch_fill_w<unsigned long long>(vs, mask_bytes, r, nrows, nbr, d_rowptr, d_colidx, d_values_in,
+                                                 d_rowsSize, nb, d_headers_out, d_values_out, s);
+    case 4:
+        return launch_fill_w<unsigned int>(vs, mask_bytes, r, nrows, nbr, d_rowptr, d_colidx, d_values_in,
+                                           d_rowsSize, nb, d_headers_out, d_values_out, s);
+    default:
+        return launch_fill_w<unsigned short>(vs, mask_bytes, r, nrows, nbr, d_rowptr, d_colidx, d_values_in,
+                                             d_rowsSize, nb, d_headers_out, d_values_out, s);
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,184 @@
+// spc5_common.hpp -- shared device helpers and host plumbing for the
+// MI355X (gfx950) SPC5 library.  Wave64 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/spc5_hip.h"
+
+namespace spc5 {
+
+// --------------------------------------------------------------------------
+// Host-side error plumbing.  Every C-ABI entry point returns a status code
+// and records a message retrievable with spc5_hip_last_error().
+// --------------------------------------------------------------------------
+void set_error(const std::string& msg);
+std::string hip_msg(hipError_t e, const char* what);
+
+#define SPC5_HIP_TRY(expr)                                                   \
+    do {                                                                     \
+        hipError_t e__ = (expr);                                             \
+        if (e__ != hipSuccess) {                                             \
+            ::spc5::set_error(::spc5::hip_msg(e__, #expr));                  \
+            return SPC5_HIP_ERR_HIP;                                         \
+        }                                                                    \
+    } while (0)
+
+#define SPC5_CHECK_ARG(cond, msg)                                            \
+    do {                                                                     \
+        if (!(cond)) {                                                       \
+            ::spc5::set_error(std::string("invalid argument: ") + (msg));    \
+            return SPC5_HIP_ERR_INVALID;                                     \
+        }                                                                    \
+    } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// --------------------------------------------------------------------------
+// Device helpers (wave64, gfx950)
+// --------------------------------------------------------------------------
+constexpr int kWave = 64;
+
+// DPP control codes (GCN/CDNA "row_shr:n").
+constexpr int kDppRowShr1 = 0x111;
+constexpr int kDppRowShr2 = 0x112;
+constexpr int kDppRowShr4 = 0x114;
+constexpr int kDppRowShr8 = 0x118;
+
+__device__ __forceinline__ int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0)); }
+
+// Compiler + memory ordering point for LDS traffic exchanged between the
+// lanes of one wave.  LDS instructions of a wave execute in order, so this
+// only has to stop the compiler from moving accesses across it.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Inclusive prefix sum of an int over the 64 lanes: DPP shifts inside each
+// 16-lane row, then the three row totals are added with scalar readlanes.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr1, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr2, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr4, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr8, 0xf, 0xf, true);
+    const int r0 = __builtin_amdgcn_readlane(v, 15);
+    const int r1 = __builtin_amdgcn_readlane(v, 31);
+    const int r2 = __builtin_amdgcn_readlane(v, 47);
+    const int row = lane_id() >> 4;
+    v += (row >= 1 ? r0 : 0) + (row >= 2 ? r1 : 0) + (row >= 3 ? r2 : 0);
+    return v;
+}
+
+__device__ __forceinline__ int wave_reduce_add(int v) {
+    return __builtin_amdgcn_readlane(wave_incl_scan(v), 63);
+}
+
+__device__ __forceinline__ unsigned long long ballot64(bool p) { return __ballot(p); }
+
+// Unaligned little-endian loads from the packed header stream.  gfx950
+// supports unaligned global accesses; the compiler emits a single dword (or
+// short/byte) load for these.
+__device__ __forceinline__ int load_i32_unaligned(const unsigned char* p) {
+    int v;
+    __builtin_memcpy(&v, p, 4);
+    return v;
+}
+
+template <int NBYTES>
+struct MaskWords;  // combined r-row mask, row s at bits [s*VS, (s+1)*VS)
+
+template <>
+struct MaskWords<1> {
+    static constexpr int kWords = 1;
+    __device__ __forceinline__ static void load(const unsigned char* p, unsigned long long* w) {
+        unsigned char v;
+        __builtin_memcpy(&v, p, 1);
+        w[0] = v;
+    }
+};
+template <>
+struct MaskWords<2> {
+    static constexpr int kWords = 1;
+    __device__ __forceinline__ static void load(const unsigned char* p, unsigned long long* w) {
+        unsigned short v;
+        __builtin_memcpy(&v, p, 2);
+        w[0] = v;
+    }
+};
+template <>
+struct MaskWords<4> {
+    static constexpr int kWords = 1;
+    __device__ __forceinline__ static void load(const unsigned char* p, unsigned long long* w) {
+        unsigned int v;
+        __builtin_memcpy(&v, p, 4);
+        w[0] = v;
+    }
+};
+template <>
+struct MaskWords<8> {
+    static constexpr int kWords = 1;
+    __device__ __forceinline__ static void load(const unsigned char* p, unsigned long long* w) {
+        unsigned long long v;
+        __builtin_memcpy(&v, p, 8);
+        w[0] = v;
+    }
+};
+template <>
+struct MaskWords<16> {
+    static constexpr int kWords = 2;
+    __device__ __forceinline__ static void load(const unsigned char* p, unsigned long long* w) {
+        unsigned long long v[2];
+        __builtin_memcpy(v, p, 16);
+        w[0] = v[0];
+        w[1] = v[1];
+    }
+};
+
+// bf16 stored as raw uint16 (upper half of an IEEE binary32).
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+template <class V, class ACC>
+struct ValueLoad {
+    __device__ __forceinline__ static ACC get(const V* p) { return (ACC)*p; }
+};
+template <>
+struct ValueLoad<unsigned short, float> {
+    __device__ __forceinline__ static float get(const unsigned short* p) { return bf16_to_f32(*p); }
+};
+
+// --------------------------------------------------------------------------
+// Host helpers shared by the translation units
+// --------------------------------------------------------------------------
+int scan_exclusive_i32(int* d_data, int64_t n, int* d_total, hipStream_t s);  // in place, n entries
+
+struct DeviceBuffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    hipError_t alloc(size_t b) {
+        release();
+        bytes = b;
+        return hipMalloc(&p, b > 0 ? b : 16);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+inline int header_bytes(int r, int mb) { return 4 + r * mb; }
+
+}  // namespace spc5

@@ -1,0 +1,423 @@
+// spmv.hip -- beta(r,VS) SpMV on gfx950: y += A x with A in the reference
+// block layout, plus the plan builder and the handle API.
+//
+// Kernel design (one wave = one tile of whole block-rows):
+//   * the tile's block-row pointers go to LDS; a running value offset walks
+//     the value stream exactly like the reference's `values += popcount`
+//     (src/spc5.hpp:2571-2617), starting from the tile's planned offset;
+//   * batch step: lane i decodes block b+i (unaligned int32 column + r
+//     masks), counts its values (popcount), and a DPP wave scan gives each
+//     block's position in the batch; blocks are taken while the batch holds
+//     at most kExpandCap values;
+//   * each taken lane finds its block-row by binary search in LDS and
+//     expands its set bits into (column, local row) pairs in LDS -- values
+//     are never zero-padded;
+//   * the batch's values are then read with fully coalesced loads, one per
+//     lane, multiplied by x[column] and added into per-row LDS accumulators;
+//   * at the end of the tile every row does one y[row] += sum (rows of a
+//     tile belong to it alone, so no global atomics and no second pass).
+#include "spc5_mat.hpp"
+
+#include <mutex>
+#include <vector>
+
+namespace spc5 {
+
+namespace {
+
+std::mutex g_err_mu;
+std::string g_err;
+
+template <int R>
+__device__ __forceinline__ int find_blockrow(const int* rs, int nbr, int top, int jr) {
+    // Largest i in [0, nbr) with rs[i] <= jr (rs[0] == 0 <= jr).
+    int lo = 0;
+    for (int step = top; step >= 1; step >>= 1) {
+        const int c = lo + step;
+        if (c < nbr && rs[c] <= jr) lo = c;
+    }
+    return lo;
+}
+
+template <class V, class X, class ACC, int R, int VS, int MB>
+__global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_tiles(
+    const int* __restrict__ bptr, const unsigned char* __restrict__ hdr, const V* __restrict__ val,
+    const X* __restrict__ x, X* __restrict__ y, const int* __restrict__ tile_br,
+    const int* __restrict__ tile_val, int64_t ntiles, int64_t nrows) {
+    constexpr int H = 4 + R * MB;
+    constexpr int kMaxBr = kMaxRows / R;
+    using MW = MaskWords<R * MB>;
+
+    __shared__ ACC s_acc[kWavesPerGroup][kMaxRows];
+    __shared__ int s_rs[kWavesPerGroup][kMaxBr + 1];
+    __shared__ int s_col[kWavesPerGroup][kExpandCap];
+    __shared__ unsigned short s_row[kWavesPerGroup][kExpandCap];
+
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int64_t t = (int64_t)blockIdx.x * kWavesPerGroup + w;
+    if (t >= ntiles) return;  // whole wave leaves; no workgroup barrier is used
+    ACC* acc = s_acc[w];
+    int* rs = s_rs[w];
+    int* ecol = s_col[w];
+    unsigned short* erow = s_row[w];
+
+    const int br0 = tile_br[t];
+    const int br1 = tile_br[t + 1];
+    const int nbr = br1 - br0;
+    const int b0 = bptr[br0];
+    const int b1 = bptr[br1];
+    int v = tile_val[t];
+    const int64_t row0 = (int64_t)br0 * R;
+    const int64_t rows_left = nrows - row0;
+    const int nrow_t = (int)((int64_t)nbr * R < rows_left ? (int64_t)nbr * R : rows_left);
+
+    for (int i = lane; i <= nbr; i += 64) rs[i] = bptr[br0 + i] - b0;
+    for (int i = lane; i < nbr * R; i += 64) acc[i] = (ACC)0;
+    int top = 1;
+    while (top < nbr) top <<= 1;
+    wave_sync();
+
+    for (int b = b0; b < b1;) {
+        const int j = b + lane;
+        const bool valid = j < b1;
+        int cs = 0;
+        unsigned long long mw[MW::kWords];
+#pragma unroll
+        for (int k = 0; k < MW::kWords; ++k) mw[k] = 0ull;
+        if (valid) {
+            const unsigned char* h = hdr + (int64_t)j * H;
+            cs = load_i32_unaligned(h);
+            MW::load(h + 4, mw);
+        }
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < MW::kWords; ++k) cnt += __popcll(mw[k]);
+        const int incl = wave_incl_scan(cnt);
+        const unsigned long long take = ballot64(valid && incl <= kExpandCap);
+        const int ntake = __popcll(take);
+        const int total = __builtin_amdgcn_readlane(incl, ntake - 1);
+        if (lane < ntake) {
+            const int rbase = find_blockrow<R>(rs, nbr, top, j - b0) * R;
+            int p = incl - cnt;
+#pragma unroll
+            for (int wd = 0; wd < MW::kWords; ++wd) {
+                unsigned long long m = mw[wd];
+                while (m) {
+                    const int bit = __builtin_ctzll(m) + wd * 64;
+                    ecol[p] = cs + (bit & (VS - 1));
+                    erow[p] = (unsigned short)(rbase + bit / VS);
+                    ++p;
+                    m &= m - 1ull;
+                }
+            }
+        }
+        wave_sync();
+        for (int q = lane; q < total; q += 64) {
+            const ACC a = ValueLoad<V, ACC>::get(val + v + q);
+            const ACC xv = (ACC)x[ecol[q]];
+            atomicAdd(&acc[erow[q]], a * xv);
+        }
+        v += total;
+        b += ntake;
+        wave_sync();
+    }
+    for (int i = lane; i < nrow_t; i += 64) y[row0 + i] = (X)((ACC)y[row0 + i] + acc[i]);
+}
+
+// ---- plan kernels --------------------------------------------------------
+
+// Checks the block-row pointers: 0-based, non-decreasing, ending at nblocks.
+__global__ void k_check_bptr(const int* __restrict__ bptr, int64_t nbr, int64_t nblocks, int* err) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nbr) return;
+    const int v = bptr[i];
+    bool bad = v < 0 || (int64_t)v > nblocks;
+    if (i == 0) bad |= v != 0;
+    if (i == nbr) bad |= (int64_t)v != nblocks;
+    if (i < nbr) bad |= bptr[i + 1] < v;
+    if (bad) atomicOr(err, 1);
+}
+
+// Values per block-row (popcount of every mask) and a column range check.
+template <int R, int MB, int VS>
+__global__ void k_blockrow_values(const int* __restrict__ bptr, const unsigned char* __restrict__ hdr,
+                                  int64_t nbr, int64_t ncols, int* __restrict__ cnt_out, int* err) {
+    constexpr int H = 4 + R * MB;
+    using MW = MaskWords<R * MB>;
+    const int64_t br = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (br >= nbr) return;
+    int c = 0;
+    bool bad = false;
+    for (int b = bptr[br]; b < bptr[br + 1]; ++b) {
+        const unsigned char* h = hdr + (int64_t)b * H;
+        const int cs = load_i32_unaligned(h);
+        unsigned long long mw[MW::kWords];
+        MW::load(h + 4, mw);
+        unsigned rowor = 0;
+        for (int k = 0; k < MW::kWords; ++k) {
+            c += __popcll(mw[k]);
+            for (int s = 0; s < 64 / VS; ++s) rowor |= (unsigned)((mw[k] >> (s * VS)) & ((1ull << VS) - 1ull));
+        }
+        if (rowor != 0) {
+            const int hi = 31 - __builtin_clz(rowor);
+            bad |= cs < 0 || (int64_t)cs + hi >= ncols;
+        }
+    }
+    cnt_out[br] = c;
+    if (bad) atomicOr(err, 2);
+}
+
+__device__ __forceinline__ bool tile_starts(const int* valptr, int64_t br, int target, int maxbr) {
+    if (br % maxbr == 0) return true;
+    return valptr[br] / target != valptr[br - 1] / target;
+}
+
+__global__ void k_tile_flags(const int* __restrict__ valptr, int64_t nbr, int target, int maxbr,
+                             int* __restrict__ flags) {
+    const int64_t br = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (br >= nbr) return;
+    flags[br] = tile_starts(valptr, br, target, maxbr) ? 1 : 0;
+}
+
+__global__ void k_tile_scatter(const int* __restrict__ valptr, int64_t nbr, int target, int maxbr,
+                               const int* __restrict__ idx, int64_t ntiles, int* __restrict__ tile_br,
+                               int* __restrict__ tile_val) {
+    const int64_t br = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (br > nbr) return;
+    if (br == nbr) {
+        tile_br[ntiles] = (int)nbr;
+        tile_val[ntiles] = valptr[nbr];
+        return;
+    }
+    if (tile_starts(valptr, br, target, maxbr)) {
+        tile_br[idx[br]] = (int)br;
+        tile_val[idx[br]] = valptr[br];
+    }
+}
+
+template <int R, int MB, int VS>
+void launch_blockrow_values(const spc5_hip_mat& m, int* cnt, int* err, hipStream_t s) {
+    const unsigned g = (unsigned)((m.nbr + 255) / 256);
+    hipLaunchKernelGGL((k_blockrow_values<R, MB, VS>), dim3(g), dim3(256), 0, s, m.bptr, m.hdr, m.nbr, m.ncols, cnt,
+                       err);
+}
+
+template <class V, class X, class ACC, int VS, int MB>
+int launch_spmv_vs(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s) {
+    const unsigned grid = (unsigned)((m.plan.ntiles + kWavesPerGroup - 1) / kWavesPerGroup);
+    const dim3 blk(kWavesPerGroup * 64);
+    const int* tb = m.plan.tile_br->as<int>();
+    const int* tv = m.plan.tile_val->as<int>();
+    const V* val = reinterpret_cast<const V*>(m.values);
+    const X* xx = reinterpret_cast<const X*>(x);
+    X* yy = reinterpret_cast<X*>(y);
+    switch (m.r) {
+    case 1:
+        hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 1, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
+                           tb, tv, m.plan.ntiles, m.nrows);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 2, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
+                           tb, tv, m.plan.ntiles, m.nrows);
+        break;
+    case 4:
+        hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 4, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
+                           tb, tv, m.plan.ntiles, m.nrows);
+        break;
+    case 8:
+        hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 8, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
+                           tb, tv, m.plan.ntiles, m.nrows);
+        break;
+    default:
+        set_error("invalid r");
+        return SPC5_HIP_ERR_INVALID;
+    }
+    SPC5_HIP_TRY(hipGetLastError());
+    return SPC5_HIP_OK;
+}
+
+}  // namespace
+
+void set_error(const std::string& msg) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = msg;
+}
+
+std::string hip_msg(hipError_t e, const char* what) {
+    return std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+}
+
+const char* last_error_cstr() {
+    static thread_local std::string copy;
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    copy = g_err;
+    return copy.c_str();
+}
+
+int launch_spmv(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s) {
+    if (m.plan.ntiles == 0) return SPC5_HIP_OK;
+    if (m.value_type == SPC5_HIP_F64) return launch_spmv_vs<double, double, double, 8, 1>(m, x, y, s);
+    if (m.value_type == SPC5_HIP_F32 && m.vs == 16) return launch_spmv_vs<float, float, float, 16, 2>(m, x, y, s);
+    if (m.value_type == SPC5_HIP_F32 && m.vs == 8) return launch_spmv_vs<float, float, float, 8, 1>(m, x, y, s);
+    if (m.value_type == SPC5_HIP_BF16) return launch_spmv_vs<unsigned short, float, float, 8, 1>(m, x, y, s);
+    set_error("unsupported value type / VS");
+    return SPC5_HIP_ERR_INVALID;
+}
+
+int build_plan(spc5_hip_mat& m, hipStream_t s) {
+    Plan& p = m.plan;
+    p.ntiles = 0;
+    if (m.nbr == 0) return SPC5_HIP_OK;
+    DeviceBuffer err;
+    SPC5_HIP_TRY(err.alloc(sizeof(int)));
+    SPC5_HIP_TRY(hipMemsetAsync(err.p, 0, sizeof(int), s));
+    {
+        const unsigned g = (unsigned)((m.nbr + 1 + 255) / 256);
+        hipLaunchKernelGGL(k_check_bptr, dim3(g), dim3(256), 0, s, m.bptr, m.nbr, m.nblocks, err.as<int>());
+        SPC5_HIP_TRY(hipGetLastError());
+        int herr = 0;
+        SPC5_HIP_TRY(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        SPC5_HIP_TRY(hipStreamSynchronize(s));
+        if (herr) {
+            set_error("rowsSize is not a 0-based non-decreasing block pointer array ending at nblocks");
+            return SPC5_HIP_ERR_INVALID;
+        }
+    }
+    p.valptr = std::make_shared<DeviceBuffer>();
+    SPC5_HIP_TRY(p.valptr->alloc(sizeof(int) * (size_t)(m.nbr + 1)));
+    int* valptr = p.valptr->as<int>();
+    SPC5_HIP_TRY(hipMemsetAsync(valptr + m.nbr, 0, sizeof(int), s));
+    const int key = m.r * 100 + m.mb * 10 + (m.vs == 16 ? 1 : 0);
+    switch (key) {
+    case 110: launch_blockrow_values<1, 1, 8>(m, valptr, err.as<int>(), s); break;
+    case 210: launch_blockrow_values<2, 1, 8>(m, valptr, err.as<int>(), s); break;
+    case 410: launch_blockrow_values<4, 1, 8>(m, valptr, err.as<int>(), s); break;
+    case 810: launch_blockrow_values<8, 1, 8>(m, valptr, err.as<int>(), s); break;
+    case 121: launch_blockrow_values<1, 2, 16>(m, valptr, err.as<int>(), s); break;
+    case 221: launch_blockrow_values<2, 2, 16>(m, valptr, err.as<int>(), s); break;
+    case 421: launch_blockrow_values<4, 2, 16>(m, valptr, err.as<int>(), s); break;
+    case 821: launch_blockrow_values<8, 2, 16>(m, valptr, err.as<int>(), s); break;
+    default: set_error("unsupported (r, VS, mask bytes)"); return SPC5_HIP_ERR_INVALID;
+    }
+    SPC5_HIP_TRY(hipGetLastError());
+    DeviceBuffer dtotal;
+    SPC5_HIP_TRY(dtotal.alloc(sizeof(int)));
+    // Counts are < 2^31 each; the scan is checked against nnz below.
+    int rc = scan_exclusive_i32(valptr, m.nbr + 1, dtotal.as<int>(), s);
+    if (rc) return rc;
+    int herr = 0, hnnz = 0;
+    SPC5_HIP_TRY(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPC5_HIP_TRY(hipMemcpyAsync(&hnnz, valptr + m.nbr, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPC5_HIP_TRY(hipStreamSynchronize(s));
+    if (herr) {
+        set_error("a block touches a column outside [0, ncols)");
+        return SPC5_HIP_ERR_INVALID;
+    }
+    if ((int64_t)hnnz != m.nnz) {
+        set_error("nnz (" + std::to_string(m.nnz) + ") differs from the mask popcount (" + std::to_string(hnnz) + ")");
+        return SPC5_HIP_ERR_INVALID;
+    }
+    // Tile cut: value target per tile, and at most kMaxRows rows per tile.
+    int64_t target = m.nnz / 8192;
+    if (target < 512) target = 512;
+    if (target > 8192) target = 8192;
+    p.target = (int)target;
+    const int maxbr = kMaxRows / m.r;
+    DeviceBuffer flags;
+    SPC5_HIP_TRY(flags.alloc(sizeof(int) * (size_t)(m.nbr + 1)));
+    const unsigned g = (unsigned)((m.nbr + 1 + 255) / 256);
+    hipLaunchKernelGGL(k_tile_flags, dim3(g), dim3(256), 0, s, valptr, m.nbr, p.target, maxbr, flags.as<int>());
+    SPC5_HIP_TRY(hipGetLastError());
+    rc = scan_exclusive_i32(flags.as<int>(), m.nbr, dtotal.as<int>(), s);
+    if (rc) return rc;
+    int hn = 0;
+    SPC5_HIP_TRY(hipMemcpyAsync(&hn, dtotal.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPC5_HIP_TRY(hipStreamSynchronize(s));
+    p.ntiles = hn;
+    p.tile_br = std::make_shared<DeviceBuffer>();
+    p.tile_val = std::make_shared<DeviceBuffer>();
+    SPC5_HIP_TRY(p.tile_br->alloc(sizeof(int) * (size_t)(hn + 1)));
+    SPC5_HIP_TRY(p.tile_val->alloc(sizeof(int) * (size_t)(hn + 1)));
+    hipLaunchKernelGGL(k_tile_scatter, dim3(g), dim3(256), 0, s, valptr, m.nbr, p.target, maxbr, flags.as<int>(),
+                       (int64_t)hn, p.tile_br->as<int>(), p.tile_val->as<int>());
+    SPC5_HIP_TRY(hipGetLastError());
+    SPC5_HIP_TRY(hipStreamSynchronize(s));
+    return SPC5_HIP_OK;
+}
+
+}  // namespace spc5
+
+using namespace spc5;
+
+extern "C" {
+
+const char* spc5_hip_last_error(void) { return spc5::last_error_cstr(); }
+
+const char* spc5_hip_version(void) { return "spc5_hip 0.1 (gfx950)"; }
+
+int spc5_hip_mat_create(int value_type, int vs, int r, int64_t nrows, int64_t ncols, int64_t nnz, int64_t nblocks,
+                        const int* d_rowsSize, const unsigned char* d_headers, const void* d_values, void* stream,
+                        spc5_hip_mat_t* out) {
+    SPC5_CHECK_ARG(out != nullptr, "out is null");
+    *out = nullptr;
+    const int mb = (vs == 16) ? 2 : 1;
+    SPC5_CHECK_ARG(r == 1 || r == 2 || r == 4 || r == 8, "r must be 1, 2, 4 or 8");
+    SPC5_CHECK_ARG(valid_format(value_type, vs, mb), "unsupported (value type, VS) combination");
+    SPC5_CHECK_ARG(nrows >= 0 && ncols >= 0 && nnz >= 0 && nblocks >= 0, "negative size");
+    SPC5_CHECK_ARG(nnz <= INT32_MAX && nblocks <= INT32_MAX && nrows <= INT32_MAX && ncols <= INT32_MAX,
+                   "sizes must fit the 32-bit format");
+    SPC5_CHECK_ARG(nrows == 0 || d_rowsSize != nullptr, "rowsSize is null");
+    SPC5_CHECK_ARG(nblocks == 0 || d_headers != nullptr, "headers are null");
+    SPC5_CHECK_ARG(nnz == 0 || d_values != nullptr, "values are null");
+    std::unique_ptr<spc5_hip_mat> m(new spc5_hip_mat());
+    m->value_type = value_type;
+    m->vs = vs;
+    m->mb = mb;
+    m->r = r;
+    m->nrows = nrows;
+    m->ncols = ncols;
+    m->nnz = nnz;
+    m->nblocks = nblocks;
+    m->nbr = (nrows + r - 1) / r;
+    m->bptr = d_rowsSize;
+    m->hdr = d_headers;
+    m->values = d_values;
+    const int rc = build_plan(*m, as_stream(stream));
+    if (rc) return rc;
+    *out = m.release();
+    return SPC5_HIP_OK;
+}
+
+int spc5_hip_mat_create_values_view(spc5_hip_mat_t base, int value_type, const void* d_values, spc5_hip_mat_t* out) {
+    SPC5_CHECK_ARG(base != nullptr && out != nullptr, "null handle");
+    SPC5_CHECK_ARG(valid_format(value_type, base->vs, base->mb), "value type incompatible with the block format");
+    SPC5_CHECK_ARG(base->nnz == 0 || d_values != nullptr, "values are null");
+    std::unique_ptr<spc5_hip_mat> m(new spc5_hip_mat(*base));
+    m->value_type = value_type;
+    m->values = d_values;
+    *out = m.release();
+    return SPC5_HIP_OK;
+}
+
+int spc5_hip_mat_destroy(spc5_hip_mat_t h) {
+    delete h;
+    return SPC5_HIP_OK;
+}
+
+int spc5_hip_mat_info(spc5_hip_mat_t h, int64_t* ntiles, int64_t* nblockrows, int64_t* nblocks, int64_t* nnz) {
+    SPC5_CHECK_ARG(h != nullptr, "null handle");
+    if (ntiles) *ntiles = h->plan.ntiles;
+    if (nblockrows) *nblockrows = h->nbr;
+    if (nblocks) *nblocks = h->nblocks;
+    if (nnz) *nnz = h->nnz;
+    return SPC5_HIP_OK;
+}
+
+int spc5_hip_spmv(spc5_hip_mat_t h, const void* d_x, void* d_y, void* stream) {
+    SPC5_CHECK_ARG(h != nullptr, "null handle");
+    SPC5_CHECK_ARG(h->nnz == 0 || (d_x != nullptr && d_y != nullptr), "x or y is null");
+    return launch_spmv(*h, d_x, d_y, as_stream(stream));
+}
+
+}  // extern "C"
