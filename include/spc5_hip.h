@@ -152,6 +152,39 @@ int spc5_hip_convert_fill(int value_bytes, int vs, int mask_bytes, int r, int64_
                           const int* d_rowsSize, unsigned char* d_headers_out, void* d_values_out,
                           void* stream);
 
+/* ---------------------------------------------------------------------- */
+/* 3. Host-pointer helpers (used by include/spc5_hip.hpp)                 */
+/* ---------------------------------------------------------------------- */
+
+/* Block count of a host CSR matrix, computed on the GPU.
+ * Replaces SPC5_<r>rVc_block_count (src/spc5.hpp:276-317, 510-513,
+ * 635-638, 764-767, 890-893). */
+int spc5_hip_host_block_count(int vs, int r, int64_t nrows, int64_t nnz, const int* rowptr, const int* colidx,
+                              int64_t* nblocks);
+
+/* Host CSR -> beta(r,VS) host arrays, converted on the GPU (byte-identical
+ * to the reference).  rowsSize_out receives ceil(N/r)+1 entries; the header
+ * buffer must hold *nblocks*(4+r*MB) bytes (the call fails, reporting the
+ * size needed, when headers_capacity is smaller).
+ * Replaces core_CSR_to_SPC5_rVc / CSR_to_SPC5_1rVc (src/spc5.hpp:165-228,
+ * 455-488). */
+int spc5_hip_host_convert(int value_bytes, int vs, int mask_bytes, int r, int64_t nrows, int64_t nnz,
+                          const int* rowptr, const int* colidx, const void* values_in, int* rowsSize_out,
+                          unsigned char* headers_out, int64_t headers_capacity, void* values_out,
+                          int64_t* nblocks);
+
+/* Device-resident copy of host arrays in the reference layout, with its
+ * plan; the handle owns the device copies (freed by spc5_hip_mat_destroy).
+ * Counterpart of SPC5_<r>rVc_split_omp (src/spc5.hpp:551-554 etc.): the
+ * one-off preparation before repeated products. */
+int spc5_hip_mat_create_from_host(int value_type, int vs, int r, int64_t nrows, int64_t ncols, int64_t nnz,
+                                  int64_t nblocks, const int* rowsSize, const unsigned char* headers,
+                                  const void* values, spc5_hip_mat_t* out);
+
+/* y += A x with host x (ncols entries) and y (nrows entries).
+ * Counterpart of SPC5_<r>rVc_Spmv_omp (src/spc5.hpp:557-599 etc.). */
+int spc5_hip_spmv_host(spc5_hip_mat_t h, const void* x, void* y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-HIP_SO = os.path.join(HERE, "libspc5_hip.so")
+# SPC5_HIP_LIB selects another build of the library (e.g. a tuning variant).
+HIP_SO = os.environ.get("SPC5_HIP_LIB") or os.path.join(HERE, "libspc5_hip.so")
 
 SPC5_HIP_OK = 0
 SPC5_HIP_F64 = 0
@@ -38,6 +39,10 @@ SIGNATURES = {
     "spc5_hip_spmv": (_I, [_P, _P, _P, _P]),
     "spc5_hip_convert_count": (_I, [_I, _I, _I64, _P, _P, _P, C.POINTER(_I64), _P]),
     "spc5_hip_convert_fill": (_I, [_I, _I, _I, _I, _I64, _P, _P, _P, _P, _P, _P, _P]),
+    "spc5_hip_host_block_count": (_I, [_I, _I, _I64, _I64, _P, _P, C.POINTER(_I64)]),
+    "spc5_hip_host_convert": (_I, [_I, _I, _I, _I, _I64, _I64, _P, _P, _P, _P, _P, _I64, _P, C.POINTER(_I64)]),
+    "spc5_hip_mat_create_from_host": (_I, [_I, _I, _I, _I64, _I64, _I64, _I64, _P, _P, _P, C.POINTER(_P)]),
+    "spc5_hip_spmv_host": (_I, [_P, _P, _P]),
 }
 
 _lib = None

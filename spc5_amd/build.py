@@ -42,23 +42,26 @@ def _run(cmd):
     return out
 
 
-def build_hip(verbose=False):
-    os.makedirs(BUILD, exist_ok=True)
+def build_hip(verbose=False, out=HIP_SO, defines=()):
+    """Compile every csrc/*.hip and link `out`.  `defines` are extra -D
+    options (tuning constants of spc5_mat.hpp / spc5_common.hpp)."""
+    bdir = BUILD if not defines else os.path.join(BUILD, "v_" + "_".join(d.replace("=", "") for d in defines))
+    os.makedirs(bdir, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     objs = []
     cmds = []
     for s in srcs:
-        o = os.path.join(BUILD, os.path.basename(s) + ".o")
+        o = os.path.join(bdir, os.path.basename(s) + ".o")
         objs.append(o)
-        cmds.append([HIPCC, *HIP_FLAGS, "-c", s, "-o", o])
+        cmds.append([HIPCC, *HIP_FLAGS, *["-D" + d for d in defines], "-c", s, "-o", o])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
-        for out in ex.map(_run, cmds):
-            if verbose and out.strip():
-                print(out)
-    tmp = HIP_SO + ".tmp"
+        for msg in ex.map(_run, cmds):
+            if verbose and msg.strip():
+                print(msg)
+    tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,-Bsymbolic", "-o", tmp, *objs])
-    os.replace(tmp, HIP_SO)
-    return HIP_SO
+    os.replace(tmp, out)
+    return out
 
 
 def build_synth(verbose=False):

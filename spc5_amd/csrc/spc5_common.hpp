@@ -76,6 +76,52 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 
+// DPP moves of 32/64-bit values inside 16-lane rows (0 where the source
+// lane is outside the row).
+constexpr int kDppRowShl1 = 0x101;
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, true);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_val(float v) {
+    return __int_as_float(dpp_i32<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_val(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i32<CTRL>((int)b);
+    const int hi = dpp_i32<CTRL>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Segmented inclusive sum inside each 16-lane row: lanes with equal `key`
+// in adjacent positions form a run; afterwards the last lane of every run
+// (inside its row) holds the run's sum.  Returns true on those lanes.
+template <class T>
+__device__ __forceinline__ bool row_segmented_sum(T& s, int key) {
+    const int lr = lane_id() & 15;
+    const int kprev = dpp_i32<kDppRowShr1>(key);
+    const int knext = dpp_i32<kDppRowShl1>(key);
+    int f = (lr == 0 || kprev != key) ? 1 : 0;  // run starts here
+#define SPC5_SEG_STEP(CTRL)                          \
+    {                                                \
+        const T su = dpp_val<CTRL>(s);               \
+        const int fu = dpp_i32<CTRL>(f);             \
+        if (!f) {                                    \
+            s += su;                                 \
+            f = fu;                                  \
+        }                                            \
+    }
+    SPC5_SEG_STEP(kDppRowShr1)
+    SPC5_SEG_STEP(kDppRowShr2)
+    SPC5_SEG_STEP(kDppRowShr4)
+    SPC5_SEG_STEP(kDppRowShr8)
+#undef SPC5_SEG_STEP
+    return lr == 15 || knext != key;
+}
+
 __device__ __forceinline__ int wave_reduce_add(int v) {
     return __builtin_amdgcn_readlane(wave_incl_scan(v), 63);
 }
@@ -144,13 +190,26 @@ struct MaskWords<16> {
 // bf16 stored as raw uint16 (upper half of an IEEE binary32).
 __device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
+// Matrix values are read once per product: non-temporal loads keep them
+// from evicting the reused x window from L2.
+#ifndef SPC5_NT_VALUES
+#define SPC5_NT_VALUES 1
+#endif
+template <class V>
+__device__ __forceinline__ V load_stream(const V* p) {
+#if SPC5_NT_VALUES
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
 template <class V, class ACC>
 struct ValueLoad {
-    __device__ __forceinline__ static ACC get(const V* p) { return (ACC)*p; }
+    __device__ __forceinline__ static ACC get(const V* p) { return (ACC)load_stream(p); }
 };
 template <>
 struct ValueLoad<unsigned short, float> {
-    __device__ __forceinline__ static float get(const unsigned short* p) { return bf16_to_f32(*p); }
+    __device__ __forceinline__ static float get(const unsigned short* p) { return bf16_to_f32(load_stream(p)); }
 };
 
 // --------------------------------------------------------------------------

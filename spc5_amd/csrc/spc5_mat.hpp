@@ -4,6 +4,7 @@
 #include "spc5_common.hpp"
 
 #include <memory>
+#include <vector>
 
 namespace spc5 {
 
@@ -12,9 +13,26 @@ namespace spc5 {
 // of the value target, and at least every kMaxRows/r block-rows, so every
 // tile's rows fit the wave's LDS accumulators and every tile carries about
 // the same number of values.
-constexpr int kMaxRows = 256;        // scalar rows per tile (LDS accumulators)
-constexpr int kWavesPerGroup = 4;    // independent waves per workgroup
-constexpr int kExpandCap = 512;      // values expanded into LDS per batch
+#ifndef SPC5_MAX_ROWS
+#define SPC5_MAX_ROWS 256
+#endif
+#ifndef SPC5_WAVES_PER_GROUP
+#define SPC5_WAVES_PER_GROUP 4
+#endif
+#ifndef SPC5_EXPAND_CAP
+#define SPC5_EXPAND_CAP 512
+#endif
+#ifndef SPC5_SEGREDUCE
+#define SPC5_SEGREDUCE 1
+#endif
+#ifndef SPC5_XCD_REMAP
+#define SPC5_XCD_REMAP 1
+#endif
+constexpr int kMaxRows = SPC5_MAX_ROWS;              // scalar rows per tile (LDS accumulators), <= 256
+constexpr int kWavesPerGroup = SPC5_WAVES_PER_GROUP; // independent waves per workgroup
+constexpr int kExpandCap = SPC5_EXPAND_CAP;          // values expanded into LDS per batch
+static_assert(kMaxRows <= 256, "tile-local rows are stored as bytes");
+static_assert(kExpandCap % 64 == 0 && kExpandCap >= 128, "expansion cap must hold a full block");
 
 struct Plan {
     int64_t ntiles = 0;
@@ -36,6 +54,7 @@ struct spc5_hip_mat {
     const unsigned char* hdr = nullptr;   // device, [nblocks*(4+r*mb)]
     const void* values = nullptr;         // device, [nnz]
     spc5::Plan plan;
+    std::vector<std::shared_ptr<spc5::DeviceBuffer>> owned;  // buffers the handle allocated itself
 };
 
 namespace spc5 {

@@ -17,6 +17,7 @@
 //   * at the end of the tile every row does one y[row] += sum (rows of a
 //     tile belong to it alone, so no global atomics and no second pass).
 #include "spc5_mat.hpp"
+#include "spmv_tile.hpp"
 
 #include <mutex>
 #include <vector>
@@ -28,15 +29,18 @@ namespace {
 std::mutex g_err_mu;
 std::string g_err;
 
-template <int R>
-__device__ __forceinline__ int find_blockrow(const int* rs, int nbr, int top, int jr) {
-    // Largest i in [0, nbr) with rs[i] <= jr (rs[0] == 0 <= jr).
-    int lo = 0;
-    for (int step = top; step >= 1; step >>= 1) {
-        const int c = lo + step;
-        if (c < nbr && rs[c] <= jr) lo = c;
-    }
-    return lo;
+// Workgroups are dealt to the 8 XCDs round-robin (group g runs on XCD g % 8).
+// Renumbering groups so that each XCD walks one contiguous eighth of the
+// tile list keeps the x window each XCD gathers from inside its own L2.
+__device__ __forceinline__ int64_t xcd_tile(int wave) {
+    const int64_t g = blockIdx.x;
+#if SPC5_XCD_REMAP
+    const int64_t per_xcd = gridDim.x >> 3;  // the grid is a multiple of 8
+    const int64_t gx = (g & 7) * per_xcd + (g >> 3);
+#else
+    const int64_t gx = g;
+#endif
+    return gx * kWavesPerGroup + wave;
 }
 
 template <class V, class X, class ACC, int R, int VS, int MB>
@@ -44,85 +48,11 @@ __global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_tiles(
     const int* __restrict__ bptr, const unsigned char* __restrict__ hdr, const V* __restrict__ val,
     const X* __restrict__ x, X* __restrict__ y, const int* __restrict__ tile_br,
     const int* __restrict__ tile_val, int64_t ntiles, int64_t nrows) {
-    constexpr int H = 4 + R * MB;
-    constexpr int kMaxBr = kMaxRows / R;
-    using MW = MaskWords<R * MB>;
-
-    __shared__ ACC s_acc[kWavesPerGroup][kMaxRows];
-    __shared__ int s_rs[kWavesPerGroup][kMaxBr + 1];
-    __shared__ int s_col[kWavesPerGroup][kExpandCap];
-    __shared__ unsigned short s_row[kWavesPerGroup][kExpandCap];
-
-    const int lane = threadIdx.x & 63;
+    __shared__ TileSmem<ACC, R> sm[kWavesPerGroup];
     const int w = threadIdx.x >> 6;
-    const int64_t t = (int64_t)blockIdx.x * kWavesPerGroup + w;
+    const int64_t t = xcd_tile(w);
     if (t >= ntiles) return;  // whole wave leaves; no workgroup barrier is used
-    ACC* acc = s_acc[w];
-    int* rs = s_rs[w];
-    int* ecol = s_col[w];
-    unsigned short* erow = s_row[w];
-
-    const int br0 = tile_br[t];
-    const int br1 = tile_br[t + 1];
-    const int nbr = br1 - br0;
-    const int b0 = bptr[br0];
-    const int b1 = bptr[br1];
-    int v = tile_val[t];
-    const int64_t row0 = (int64_t)br0 * R;
-    const int64_t rows_left = nrows - row0;
-    const int nrow_t = (int)((int64_t)nbr * R < rows_left ? (int64_t)nbr * R : rows_left);
-
-    for (int i = lane; i <= nbr; i += 64) rs[i] = bptr[br0 + i] - b0;
-    for (int i = lane; i < nbr * R; i += 64) acc[i] = (ACC)0;
-    int top = 1;
-    while (top < nbr) top <<= 1;
-    wave_sync();
-
-    for (int b = b0; b < b1;) {
-        const int j = b + lane;
-        const bool valid = j < b1;
-        int cs = 0;
-        unsigned long long mw[MW::kWords];
-#pragma unroll
-        for (int k = 0; k < MW::kWords; ++k) mw[k] = 0ull;
-        if (valid) {
-            const unsigned char* h = hdr + (int64_t)j * H;
-            cs = load_i32_unaligned(h);
-            MW::load(h + 4, mw);
-        }
-        int cnt = 0;
-#pragma unroll
-        for (int k = 0; k < MW::kWords; ++k) cnt += __popcll(mw[k]);
-        const int incl = wave_incl_scan(cnt);
-        const unsigned long long take = ballot64(valid && incl <= kExpandCap);
-        const int ntake = __popcll(take);
-        const int total = __builtin_amdgcn_readlane(incl, ntake - 1);
-        if (lane < ntake) {
-            const int rbase = find_blockrow<R>(rs, nbr, top, j - b0) * R;
-            int p = incl - cnt;
-#pragma unroll
-            for (int wd = 0; wd < MW::kWords; ++wd) {
-                unsigned long long m = mw[wd];
-                while (m) {
-                    const int bit = __builtin_ctzll(m) + wd * 64;
-                    ecol[p] = cs + (bit & (VS - 1));
-                    erow[p] = (unsigned short)(rbase + bit / VS);
-                    ++p;
-                    m &= m - 1ull;
-                }
-            }
-        }
-        wave_sync();
-        for (int q = lane; q < total; q += 64) {
-            const ACC a = ValueLoad<V, ACC>::get(val + v + q);
-            const ACC xv = (ACC)x[ecol[q]];
-            atomicAdd(&acc[erow[q]], a * xv);
-        }
-        v += total;
-        b += ntake;
-        wave_sync();
-    }
-    for (int i = lane; i < nrow_t; i += 64) y[row0 + i] = (X)((ACC)y[row0 + i] + acc[i]);
+    spmv_tile<V, X, ACC, R, VS, MB>(sm[w], bptr, hdr, val, x, y, tile_br[t], tile_br[t + 1], tile_val[t], nrows);
 }
 
 // ---- plan kernels --------------------------------------------------------
@@ -205,7 +135,8 @@ void launch_blockrow_values(const spc5_hip_mat& m, int* cnt, int* err, hipStream
 
 template <class V, class X, class ACC, int VS, int MB>
 int launch_spmv_vs(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s) {
-    const unsigned grid = (unsigned)((m.plan.ntiles + kWavesPerGroup - 1) / kWavesPerGroup);
+    const int64_t groups = (m.plan.ntiles + kWavesPerGroup - 1) / kWavesPerGroup;
+    const unsigned grid = (unsigned)((groups + 7) / 8 * 8);
     const dim3 blk(kWavesPerGroup * 64);
     const int* tb = m.plan.tile_br->as<int>();
     const int* tv = m.plan.tile_val->as<int>();

@@ -161,7 +161,7 @@ def test_gpu_spmv_vs_oracle(seed, tag, r, shape):
         rp, col, v = clustered_csr(2003 + seed, 3000, seed, dtype=dt, empty_frac=0.2)
         ncols = 3000
     elif shape == "tall":
-        rp, col, v = clustered_csr(5001, 97, seed, mean_len=20, dtype=dt)
+        rp, col, v = clustered_csr(1501, 97, seed, mean_len=20, dtype=dt)
         ncols = 97
     else:
         rp, col, v = clustered_csr(7, 20000, seed, mean_len=300, spread=9000, dtype=dt)
