@@ -77,9 +77,24 @@ def build_synth(verbose=False):
     return SYNTH_SO
 
 
+def build_cpp_driver(verbose=False):
+    """The reference-style C++ driver of the drop-in header (tests/drivers)."""
+    src = os.path.join(ROOT, "tests", "drivers", "spc5_compare.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "drivers", "spc5_compare")
+    msg = _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), src,
+                "-L" + PKG, "-lspc5_hip", "-Wl,-rpath,$ORIGIN/../../spc5_amd", "-o", out + ".tmp"])
+    if verbose and msg.strip():
+        print(msg)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(verbose=False):
     build_hip(verbose)
     build_synth(verbose)
+    build_cpp_driver(verbose)
 
 
 if __name__ == "__main__":
