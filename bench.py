@@ -133,6 +133,64 @@ def cpu_baseline(rp, col, val, r, ncols, nthreads, nloops=16, max_rows=None):
             "ms_per_spmv": round(tspmv / nloops * 1e3, 3)}
 
 
+def solve_benchmark(n=160, r=4, rtol=1e-10):
+    """Precision form on the 27-point stencil n^3 (diag 26.001, off -1, SPD):
+    wall time to relative residual rtol of double CG and of mixed-precision
+    IR-CG with float and bfloat16 values, plus errors against the double
+    solution and of one reduced-precision product against the double one."""
+    import numpy as np
+    import torch
+    from spc5_amd import synth
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import cg, ir_cg, lowp
+
+    rp, col, val = synth.stencil27(n)
+    N = len(rp) - 1
+    A64 = DeviceMatrix.from_csr(torch.from_numpy(rp).cuda(), torch.from_numpy(col).cuda(),
+                                torch.from_numpy(val).cuda(), r)
+    b = torch.from_numpy(synth.uniform(N, 7)).cuda()
+    out = {"workload": f"stencil27 {n}^3 (N={N}, nnz={len(val)}), diag 26.001, beta({r},8), b U[-1,1] seed 7",
+           "rtol": rtol}
+    # warm-up of every code path on a short solve
+    cg(A64, b, rtol=1e-2, maxit=16)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x64, st = cg(A64, b, rtol=rtol, maxit=20000)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    out["fp64_cg"] = {"seconds": round(t64, 4), "iterations": st["iterations"], "relres": st["relres"]}
+    log(f"[bench] fp64 CG: {t64 * 1e3:.1f} ms, {st['iterations']} it, relres {st['relres']:.2e}")
+    nx64 = float(torch.linalg.vector_norm(x64))
+    for kind in ("f32", "bf16"):
+        Alo = lowp(A64, kind)
+        ir_cg(A64, Alo, b, rtol=1e-2, max_outer=1, max_inner=16)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, st = ir_cg(A64, Alo, b, rtol=rtol)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        # one product in reduced precision against the double product
+        bf = b.to(torch.float32)
+        ylo = torch.zeros(N, dtype=torch.float32, device="cuda")
+        Alo.spmv(bf, ylo)
+        y64 = torch.zeros(N, dtype=torch.float64, device="cuda")
+        A64.spmv(bf.to(torch.float64), y64)
+        spmv_err = float((ylo.double() - y64).abs().max() / (54.0 * float(bf.abs().max())))
+        out[f"ir_cg_{kind}"] = {
+            "seconds": round(t, 4), "speedup_vs_fp64": round(t64 / t, 3), "inner_iterations": st["iterations"],
+            "outer": st["outer"], "relres": st["relres"],
+            "x_rel_diff_vs_fp64_cg": float(torch.linalg.vector_norm(x - x64)) / nx64,
+            "spmv_normwise_err_vs_fp64": spmv_err,
+            "value_bytes": Alo.values.element_size(), "bytes_per_spmv": Alo.algorithmic_bytes()}
+        log(f"[bench] IR-CG {kind}: {t * 1e3:.1f} ms ({t64 / t:.2f}x), {st['outer']} outer / "
+            f"{st['iterations']} inner, relres {st['relres']:.2e}")
+        del Alo
+    out["fp64_bytes_per_spmv"] = A64.algorithmic_bytes()
+    del A64
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +203,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-max-rows", type=int, default=0)
     ap.add_argument("--no-copy", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--solve-n", type=int, default=160)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -217,6 +277,10 @@ def main():
         "formats": formats,
         "copy_gbs_measured": copy_gbs,
     }
+    del d_rp, d_col, d_val, x, y
+    torch.cuda.empty_cache()
+    if not args.no_solve:
+        out["solve"] = solve_benchmark(args.solve_n)
     if not args.no_cpu:
         log("[bench] CPU baseline (reference OpenMP path)...")
         th = args.cpu_threads or os.cpu_count() or 1

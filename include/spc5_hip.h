@@ -152,6 +152,67 @@ int spc5_hip_convert_fill(int value_bytes, int vs, int mask_bytes, int r, int64_
                           const int* d_rowsSize, unsigned char* d_headers_out, void* d_values_out,
                           void* stream);
 
+/* y = alpha * (A x) + beta * y on device vectors (y is not read when
+ * beta == 0).  alpha = beta = 1 is spc5_hip_spmv.  No counterpart in the
+ * reference (its kernels only accumulate); used by the solvers. */
+int spc5_hip_spmv_ex(spc5_hip_mat_t h, double alpha, const void* d_x, double beta, void* d_y, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Precision form: reduced-precision values and refinement solvers        */
+/* (no counterpart in the reference).                                     */
+/* ---------------------------------------------------------------------- */
+
+typedef struct {
+    int iterations;           /* CG iterations (for IR: float CG iterations in total) */
+    int outer;                /* refinement steps (IR only) */
+    double relres;            /* ||b - A x|| / ||b||, recomputed at the end */
+    double recurrence_relres; /* reserved, 0 */
+} spc5_hip_solve_stats;
+
+/* d_out[i] = d_in[i] rounded to float (to_type F32) or to bfloat16 through
+ * float (to_type BF16, raw uint16), each step round-to-nearest-even.  Makes
+ * the value array of a values_view of a double matrix. */
+int spc5_hip_values_convert(int to_type, const double* d_in, void* d_out, int64_t n, void* stream);
+
+/* Conjugate gradient on A x = b (A symmetric positive definite), vectors
+ * in the handle's vector type, x holding the initial guess.  Stops when the
+ * recurrence residual falls to rtol*||b|| (checked every 8 iterations) or
+ * after maxit iterations; st->relres is the true residual. */
+int spc5_hip_cg(spc5_hip_mat_t A, const void* d_b, void* d_x, double rtol, int maxit, spc5_hip_solve_stats* st,
+                void* stream);
+
+/* Mixed-precision iterative refinement around CG: residuals r = b - A64 x
+ * and updates of x in double (A64: double values), correction equations
+ * A_lo d = r/||r|| solved by float CG to inner_rtol (A_lo: float or
+ * bfloat16 values, same matrix).  Stops at ||b - A64 x|| <= rtol ||b|| or
+ * after max_outer refinements; x holds the initial guess. */
+int spc5_hip_ir_cg(spc5_hip_mat_t A64, spc5_hip_mat_t Alo, const double* d_b, double* d_x, double rtol,
+                   double inner_rtol, int max_outer, int max_inner, spc5_hip_solve_stats* st, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Batching form: many independent matrices in one persistent launch      */
+/* (no counterpart in the reference).                                     */
+/* ---------------------------------------------------------------------- */
+
+typedef struct spc5_hip_batch* spc5_hip_batch_t;
+
+/* Group nmats matrix handles (one value type and VS; each its own r) with
+ * their device vectors x_i, y_i.  The tiles of all matrices form one work
+ * list walked by `waves` persistent waves (0: 16 per compute unit); the
+ * launch is captured once into a hipGraph.  Handles and vectors are
+ * borrowed and must outlive the batch. */
+int spc5_hip_batch_create(int nmats, const spc5_hip_mat_t* mats, const void* const* d_x, void* const* d_y,
+                          int waves, spc5_hip_batch_t* out);
+
+/* y_i += A_i x_i for every matrix of the batch: one graph replay. */
+int spc5_hip_batch_spmv(spc5_hip_batch_t b, void* stream);
+
+/* Same product through a direct kernel launch (no graph). */
+int spc5_hip_batch_spmv_direct(spc5_hip_batch_t b, void* stream);
+
+int spc5_hip_batch_info(spc5_hip_batch_t b, int64_t* ntiles, int* waves);
+int spc5_hip_batch_destroy(spc5_hip_batch_t b);
+
 /* ---------------------------------------------------------------------- */
 /* 3. Host-pointer helpers (used by include/spc5_hip.hpp)                 */
 /* ---------------------------------------------------------------------- */

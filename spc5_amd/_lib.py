@@ -43,7 +43,22 @@ SIGNATURES = {
     "spc5_hip_host_convert": (_I, [_I, _I, _I, _I, _I64, _I64, _P, _P, _P, _P, _P, _I64, _P, C.POINTER(_I64)]),
     "spc5_hip_mat_create_from_host": (_I, [_I, _I, _I, _I64, _I64, _I64, _I64, _P, _P, _P, C.POINTER(_P)]),
     "spc5_hip_spmv_host": (_I, [_P, _P, _P]),
+    "spc5_hip_spmv_ex": (_I, [_P, C.c_double, _P, C.c_double, _P, _P]),
+    "spc5_hip_values_convert": (_I, [_I, _P, _P, _I64, _P]),
+    "spc5_hip_cg": (_I, [_P, _P, _P, C.c_double, _I, _P, _P]),
+    "spc5_hip_ir_cg": (_I, [_P, _P, _P, _P, C.c_double, C.c_double, _I, _I, _P, _P]),
+    "spc5_hip_batch_create": (_I, [_I, _P, _P, _P, _I, C.POINTER(_P)]),
+    "spc5_hip_batch_spmv": (_I, [_P, _P]),
+    "spc5_hip_batch_spmv_direct": (_I, [_P, _P]),
+    "spc5_hip_batch_info": (_I, [_P, C.POINTER(_I64), C.POINTER(_I)]),
+    "spc5_hip_batch_destroy": (_I, [_P]),
 }
+
+
+class SolveStats(C.Structure):
+    """spc5_hip_solve_stats"""
+    _fields_ = [("iterations", C.c_int), ("outer", C.c_int), ("relres", C.c_double),
+                ("recurrence_relres", C.c_double)]
 
 _lib = None
 

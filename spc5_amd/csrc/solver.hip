@@ -1,0 +1,334 @@
+// solver.hip -- conjugate gradient and mixed-precision iterative refinement
+// on beta(r,VS) matrices (the precision form; no counterpart in the
+// reference).
+//
+// CG iteration, three kernels, scalars kept on the device:
+//   K1  q = A p            (SpMV, y = A x epilogue, fused pq += p.q)
+//   K2  r -= alpha q       (alpha = rr / pq), rr_new += r.r
+//   K3  x += alpha p, p = r + beta p   (beta = rr_new / rr); the last
+//       workgroup to finish rotates rr <- rr_new and clears pq, rr_new.
+// The host reads rr every `check` iterations to stop.
+//
+// Iterative refinement (IR-CG): x64 = 0, r64 = b; repeat
+//   r32 = r64 / ||r64||  (float);  solve A_lo d = r32 with float CG to a
+//   loose tolerance (A_lo: the same block structure with float or bfloat16
+//   values, float accumulation);  x64 += ||r64|| d;  r64 = b - A64 x64
+//   (double SpMV) -- until ||r64|| <= rtol ||b||.
+#include "spc5_mat.hpp"
+
+#include <cmath>
+#include <vector>
+
+using namespace spc5;
+
+namespace spc5 {
+namespace {
+
+struct CgScalars {
+    double rr;
+    double rr_new;
+    double pq;
+    double pad;
+    unsigned int done;
+};
+
+constexpr int kVecThreads = 256;
+
+__device__ __forceinline__ double block_sum(double v, double* s_part) {
+    v = wave_sum_f64(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) s_part[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kVecThreads / 64; ++i) t += s_part[i];
+    return t;  // valid in thread 0
+}
+
+template <class X>
+__global__ __launch_bounds__(kVecThreads) void k_dot(const X* __restrict__ a, const X* __restrict__ b, int64_t n,
+                                                     double* out) {
+    __shared__ double s_part[kVecThreads / 64];
+    double p = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads)
+        p += (double)a[i] * (double)b[i];
+    p = block_sum(p, s_part);
+    if (threadIdx.x == 0) atomicAdd(out, p);
+}
+
+template <class X>
+__global__ __launch_bounds__(kVecThreads) void k_fill(X* __restrict__ a, int64_t n, X v) {
+    for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads)
+        a[i] = v;
+}
+
+template <class X>
+__global__ __launch_bounds__(kVecThreads) void k_copy(X* __restrict__ dst, const X* __restrict__ src, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads)
+        dst[i] = src[i];
+}
+
+// r32 = r64 / ||r64|| with ||r64||^2 read from the device.
+__global__ __launch_bounds__(kVecThreads) void k_scale_to_f32(float* __restrict__ dst, const double* __restrict__ src,
+                                                              int64_t n, const double* __restrict__ nrm2) {
+    const double inv = 1.0 / sqrt(*nrm2);
+    for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads)
+        dst[i] = (float)(src[i] * inv);
+}
+
+// x64 += ||r64|| * d32
+__global__ __launch_bounds__(kVecThreads) void k_add_scaled_f32(double* __restrict__ x, const float* __restrict__ d,
+                                                                int64_t n, const double* __restrict__ nrm2) {
+    const double s = sqrt(*nrm2);
+    for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads)
+        x[i] += s * (double)d[i];
+}
+
+// K2: r -= alpha q; rr_new += r.r
+template <class X>
+__global__ __launch_bounds__(kVecThreads) void k_cg_r(X* __restrict__ r, const X* __restrict__ q, int64_t n,
+                                                      CgScalars* sc) {
+    __shared__ double s_part[kVecThreads / 64];
+    const X alpha = (X)(sc->rr / sc->pq);
+    double p = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads) {
+        const X ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        p += (double)ri * (double)ri;
+    }
+    p = block_sum(p, s_part);
+    if (threadIdx.x == 0) atomicAdd(&sc->rr_new, p);
+}
+
+// K3: x += alpha p; p = r + beta p; the last workgroup rotates the scalars.
+template <class X>
+__global__ __launch_bounds__(kVecThreads) void k_cg_xp(X* __restrict__ x, X* __restrict__ p, const X* __restrict__ r,
+                                                       int64_t n, CgScalars* sc) {
+    __shared__ bool s_last;
+    const double rr = sc->rr;
+    const X alpha = (X)(rr / sc->pq);
+    const X beta = (X)(sc->rr_new / rr);
+    for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads) {
+        const X pi = p[i];
+        x[i] += alpha * pi;
+        p[i] = r[i] + beta * pi;
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicInc(&sc->done, gridDim.x - 1) == gridDim.x - 1;
+    __syncthreads();
+    if (s_last && threadIdx.x == 0) {
+        __threadfence();
+        sc->rr = sc->rr_new;
+        sc->rr_new = 0.0;
+        sc->pq = 0.0;
+    }
+}
+
+__global__ void k_cg_init_scalars(CgScalars* sc, const double* rr) {
+    sc->rr = *rr;
+    sc->rr_new = 0.0;
+    sc->pq = 0.0;
+    sc->pad = 0.0;
+    sc->done = 0u;
+}
+
+unsigned vec_grid(int64_t n) {
+    int64_t g = (n + kVecThreads - 1) / kVecThreads;
+    if (g > 8192) g = 8192;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+template <class X>
+int dot(const X* a, const X* b, int64_t n, double* d_out, hipStream_t s) {
+    SPC5_HIP_TRY(hipMemsetAsync(d_out, 0, sizeof(double), s));
+    hipLaunchKernelGGL(k_dot<X>, dim3(vec_grid(n)), dim3(kVecThreads), 0, s, a, b, n, d_out);
+    SPC5_HIP_TRY(hipGetLastError());
+    return SPC5_HIP_OK;
+}
+
+int read_scalar(const double* d, double* h, hipStream_t s) {
+    SPC5_HIP_TRY(hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, s));
+    SPC5_HIP_TRY(hipStreamSynchronize(s));
+    return SPC5_HIP_OK;
+}
+
+// CG on A x = b, x holds the initial guess.  rtol is relative to ||b||.
+// Vectors are of type X (double for F64 handles, float otherwise).
+template <class X>
+int cg_impl(const spc5_hip_mat& A, const X* b, X* x, double rtol, int maxit, int check, spc5_hip_solve_stats* st,
+            hipStream_t s) {
+    const int64_t n = A.nrows;
+    SPC5_CHECK_ARG(A.nrows == A.ncols, "CG needs a square matrix");
+    DeviceBuffer vr, vp, vq, vsc, vtmp;
+    SPC5_HIP_TRY(vr.alloc(sizeof(X) * (size_t)n));
+    SPC5_HIP_TRY(vp.alloc(sizeof(X) * (size_t)n));
+    SPC5_HIP_TRY(vq.alloc(sizeof(X) * (size_t)n));
+    SPC5_HIP_TRY(vsc.alloc(sizeof(CgScalars)));
+    SPC5_HIP_TRY(vtmp.alloc(2 * sizeof(double)));
+    X* r = vr.as<X>();
+    X* p = vp.as<X>();
+    X* q = vq.as<X>();
+    CgScalars* sc = vsc.as<CgScalars>();
+    double* d_bb = vtmp.as<double>();
+    double* d_rr = d_bb + 1;
+    const unsigned g = vec_grid(n);
+    int rc;
+    // r = b - A x, p = r
+    hipLaunchKernelGGL(k_copy<X>, dim3(g), dim3(kVecThreads), 0, s, r, b, n);
+    Epilogue ep;
+    ep.alpha = -1.0;
+    ep.beta = 1.0;
+    if ((rc = launch_spmv(A, x, r, s, ep))) return rc;
+    hipLaunchKernelGGL(k_copy<X>, dim3(g), dim3(kVecThreads), 0, s, p, r, n);
+    if ((rc = dot<X>(b, b, n, d_bb, s))) return rc;
+    if ((rc = dot<X>(r, r, n, d_rr, s))) return rc;
+    hipLaunchKernelGGL(k_cg_init_scalars, dim3(1), dim3(1), 0, s, sc, d_rr);
+    SPC5_HIP_TRY(hipGetLastError());
+    double hbb = 0.0, hrr = 0.0;
+    if ((rc = read_scalar(d_bb, &hbb, s))) return rc;
+    if ((rc = read_scalar(d_rr, &hrr, s))) return rc;
+    const double stop2 = rtol * rtol * (hbb > 0.0 ? hbb : 1.0);
+    int it = 0;
+    Epilogue ek1;
+    ek1.alpha = 1.0;
+    ek1.beta = 0.0;
+    ek1.dv = p;
+    ek1.dot = &sc->pq;
+    while (hrr > stop2 && it < maxit) {
+        const int chunk = (maxit - it) < check ? (maxit - it) : check;
+        for (int c = 0; c < chunk; ++c) {
+            if ((rc = launch_spmv(A, p, q, s, ek1))) return rc;
+            hipLaunchKernelGGL(k_cg_r<X>, dim3(g), dim3(kVecThreads), 0, s, r, q, n, sc);
+            hipLaunchKernelGGL(k_cg_xp<X>, dim3(g), dim3(kVecThreads), 0, s, x, p, r, n, sc);
+        }
+        SPC5_HIP_TRY(hipGetLastError());
+        it += chunk;
+        if ((rc = read_scalar(&sc->rr, &hrr, s))) return rc;
+        if (!std::isfinite(hrr)) break;
+    }
+    // true residual ||b - A x|| / ||b||
+    hipLaunchKernelGGL(k_copy<X>, dim3(g), dim3(kVecThreads), 0, s, r, b, n);
+    if ((rc = launch_spmv(A, x, r, s, ep))) return rc;
+    if ((rc = dot<X>(r, r, n, d_rr, s))) return rc;
+    if ((rc = read_scalar(d_rr, &hrr, s))) return rc;
+    if (st) {
+        st->iterations = it;
+        st->outer = 0;
+        st->relres = std::sqrt(hrr / (hbb > 0.0 ? hbb : 1.0));
+        st->recurrence_relres = 0.0;
+    }
+    return SPC5_HIP_OK;
+}
+
+// double -> float (round to nearest even), or -> bfloat16 through float
+// (each step round to nearest even; NaN stays NaN).
+__device__ __forceinline__ unsigned short f32_to_bf16_rne(float f) {
+    const unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (unsigned short)((u >> 16) | 0x40u);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__global__ __launch_bounds__(kVecThreads) void k_convert_values(int to_type, const double* __restrict__ in,
+                                                                void* __restrict__ out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads) {
+        const float f = (float)in[i];
+        if (to_type == SPC5_HIP_F32)
+            reinterpret_cast<float*>(out)[i] = f;
+        else
+            reinterpret_cast<unsigned short*>(out)[i] = f32_to_bf16_rne(f);
+    }
+}
+
+}  // namespace
+}  // namespace spc5
+
+extern "C" {
+
+int spc5_hip_values_convert(int to_type, const double* d_in, void* d_out, int64_t n, void* stream) {
+    SPC5_CHECK_ARG(to_type == SPC5_HIP_F32 || to_type == SPC5_HIP_BF16, "to_type must be F32 or BF16");
+    SPC5_CHECK_ARG(n >= 0 && (n == 0 || (d_in != nullptr && d_out != nullptr)), "null argument");
+    if (n == 0) return SPC5_HIP_OK;
+    hipLaunchKernelGGL(k_convert_values, dim3(vec_grid(n)), dim3(kVecThreads), 0, as_stream(stream), to_type, d_in,
+                       d_out, n);
+    SPC5_HIP_TRY(hipGetLastError());
+    return SPC5_HIP_OK;
+}
+
+int spc5_hip_spmv_ex(spc5_hip_mat_t h, double alpha, const void* d_x, double beta, void* d_y, void* stream) {
+    SPC5_CHECK_ARG(h != nullptr, "null handle");
+    SPC5_CHECK_ARG(d_y != nullptr && (h->nnz == 0 || d_x != nullptr), "x or y is null");
+    Epilogue ep;
+    ep.alpha = alpha;
+    ep.beta = beta;
+    return launch_spmv(*h, d_x, d_y, as_stream(stream), ep);
+}
+
+int spc5_hip_cg(spc5_hip_mat_t A, const void* d_b, void* d_x, double rtol, int maxit, spc5_hip_solve_stats* st,
+                void* stream) {
+    SPC5_CHECK_ARG(A != nullptr && d_b != nullptr && d_x != nullptr, "null argument");
+    SPC5_CHECK_ARG(rtol > 0.0 && maxit >= 0, "rtol must be > 0 and maxit >= 0");
+    hipStream_t s = as_stream(stream);
+    if (A->value_type == SPC5_HIP_F64)
+        return cg_impl<double>(*A, (const double*)d_b, (double*)d_x, rtol, maxit, 8, st, s);
+    return cg_impl<float>(*A, (const float*)d_b, (float*)d_x, rtol, maxit, 8, st, s);
+}
+
+int spc5_hip_ir_cg(spc5_hip_mat_t A64, spc5_hip_mat_t Alo, const double* d_b, double* d_x, double rtol,
+                   double inner_rtol, int max_outer, int max_inner, spc5_hip_solve_stats* st, void* stream) {
+    SPC5_CHECK_ARG(A64 != nullptr && Alo != nullptr && d_b != nullptr && d_x != nullptr, "null argument");
+    SPC5_CHECK_ARG(A64->value_type == SPC5_HIP_F64, "A64 must hold double values");
+    SPC5_CHECK_ARG(Alo->value_type != SPC5_HIP_F64, "Alo must hold float or bfloat16 values");
+    SPC5_CHECK_ARG(A64->nrows == Alo->nrows && A64->ncols == Alo->ncols && A64->nrows == A64->ncols,
+                   "A64 and Alo must be the same square matrix");
+    SPC5_CHECK_ARG(rtol > 0.0 && inner_rtol > 0.0 && inner_rtol < 1.0, "tolerances out of range");
+    hipStream_t s = as_stream(stream);
+    const int64_t n = A64->nrows;
+    DeviceBuffer vr, vr32, vd32, vtmp;
+    SPC5_HIP_TRY(vr.alloc(sizeof(double) * (size_t)n));
+    SPC5_HIP_TRY(vr32.alloc(sizeof(float) * (size_t)n));
+    SPC5_HIP_TRY(vd32.alloc(sizeof(float) * (size_t)n));
+    SPC5_HIP_TRY(vtmp.alloc(2 * sizeof(double)));
+    double* r = vr.as<double>();
+    float* r32 = vr32.as<float>();
+    float* d32 = vd32.as<float>();
+    double* d_bb = vtmp.as<double>();
+    double* d_rr = d_bb + 1;
+    const unsigned g = vec_grid(n);
+    int rc;
+    Epilogue ep;
+    ep.alpha = -1.0;
+    ep.beta = 1.0;
+    hipLaunchKernelGGL(k_copy<double>, dim3(g), dim3(kVecThreads), 0, s, r, d_b, n);
+    if ((rc = launch_spmv(*A64, d_x, r, s, ep))) return rc;
+    if ((rc = dot<double>(d_b, d_b, n, d_bb, s))) return rc;
+    double hbb = 0.0, hrr = 0.0;
+    if ((rc = read_scalar(d_bb, &hbb, s))) return rc;
+    if (hbb == 0.0) hbb = 1.0;
+    int outer = 0, inner_total = 0;
+    for (;;) {
+        if ((rc = dot<double>(r, r, n, d_rr, s))) return rc;
+        if ((rc = read_scalar(d_rr, &hrr, s))) return rc;
+        if (hrr <= rtol * rtol * hbb || outer >= max_outer || !std::isfinite(hrr)) break;
+        hipLaunchKernelGGL(k_scale_to_f32, dim3(g), dim3(kVecThreads), 0, s, r32, r, n, d_rr);
+        hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kVecThreads), 0, s, d32, n, 0.0f);
+        SPC5_HIP_TRY(hipGetLastError());
+        spc5_hip_solve_stats in{};
+        if ((rc = cg_impl<float>(*Alo, r32, d32, inner_rtol, max_inner, 8, &in, s))) return rc;
+        inner_total += in.iterations;
+        hipLaunchKernelGGL(k_add_scaled_f32, dim3(g), dim3(kVecThreads), 0, s, d_x, d32, n, d_rr);
+        hipLaunchKernelGGL(k_copy<double>, dim3(g), dim3(kVecThreads), 0, s, r, d_b, n);
+        if ((rc = launch_spmv(*A64, d_x, r, s, ep))) return rc;
+        ++outer;
+    }
+    if (st) {
+        st->iterations = inner_total;
+        st->outer = outer;
+        st->relres = std::sqrt(hrr / hbb);
+        st->recurrence_relres = 0.0;
+    }
+    return SPC5_HIP_OK;
+}
+
+}  // extern "C"

@@ -122,6 +122,12 @@ __device__ __forceinline__ bool row_segmented_sum(T& s, int key) {
     return lr == 15 || knext != key;
 }
 
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 __device__ __forceinline__ int wave_reduce_add(int v) {
     return __builtin_amdgcn_readlane(wave_incl_scan(v), 63);
 }

@@ -34,6 +34,15 @@ constexpr int kExpandCap = SPC5_EXPAND_CAP;          // values expanded into LDS
 static_assert(kMaxRows <= 256, "tile-local rows are stored as bytes");
 static_assert(kExpandCap % 64 == 0 && kExpandCap >= 128, "expansion cap must hold a full block");
 
+// Output stage of a product: y = alpha * (A x) + beta * y (y is not read
+// when beta == 0); with dv != null, sum(dv[i] * y_new[i]) is added to *dot.
+struct Epilogue {
+    double alpha = 1.0;
+    double beta = 1.0;
+    const void* dv = nullptr;
+    double* dot = nullptr;
+};
+
 struct Plan {
     int64_t ntiles = 0;
     int target = 0;                   // values per tile aimed at
@@ -59,8 +68,8 @@ struct spc5_hip_mat {
 
 namespace spc5 {
 
-// Launch y += A x for a handle (defined in spmv.hip).
-int launch_spmv(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s);
+// Launch y = alpha A x + beta y for a handle (defined in spmv.hip).
+int launch_spmv(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s, const Epilogue& ep = Epilogue());
 // Build the tile plan of a handle (defined in spmv.hip).
 int build_plan(spc5_hip_mat& m, hipStream_t s);
 

@@ -47,12 +47,13 @@ template <class V, class X, class ACC, int R, int VS, int MB>
 __global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_tiles(
     const int* __restrict__ bptr, const unsigned char* __restrict__ hdr, const V* __restrict__ val,
     const X* __restrict__ x, X* __restrict__ y, const int* __restrict__ tile_br,
-    const int* __restrict__ tile_val, int64_t ntiles, int64_t nrows) {
+    const int* __restrict__ tile_val, int64_t ntiles, int64_t nrows, Epilogue ep) {
     __shared__ TileSmem<ACC, R> sm[kWavesPerGroup];
     const int w = threadIdx.x >> 6;
     const int64_t t = xcd_tile(w);
     if (t >= ntiles) return;  // whole wave leaves; no workgroup barrier is used
-    spmv_tile<V, X, ACC, R, VS, MB>(sm[w], bptr, hdr, val, x, y, tile_br[t], tile_br[t + 1], tile_val[t], nrows);
+    spmv_tile<V, X, ACC, R, VS, MB>(sm[w], bptr, hdr, val, x, y, tile_br[t], tile_br[t + 1], tile_val[t], nrows,
+                                    ep);
 }
 
 // ---- plan kernels --------------------------------------------------------
@@ -134,7 +135,7 @@ void launch_blockrow_values(const spc5_hip_mat& m, int* cnt, int* err, hipStream
 }
 
 template <class V, class X, class ACC, int VS, int MB>
-int launch_spmv_vs(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s) {
+int launch_spmv_vs(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s, const Epilogue& ep) {
     const int64_t groups = (m.plan.ntiles + kWavesPerGroup - 1) / kWavesPerGroup;
     const unsigned grid = (unsigned)((groups + 7) / 8 * 8);
     const dim3 blk(kWavesPerGroup * 64);
@@ -146,19 +147,19 @@ int launch_spmv_vs(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s)
     switch (m.r) {
     case 1:
         hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 1, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
-                           tb, tv, m.plan.ntiles, m.nrows);
+                           tb, tv, m.plan.ntiles, m.nrows, ep);
         break;
     case 2:
         hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 2, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
-                           tb, tv, m.plan.ntiles, m.nrows);
+                           tb, tv, m.plan.ntiles, m.nrows, ep);
         break;
     case 4:
         hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 4, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
-                           tb, tv, m.plan.ntiles, m.nrows);
+                           tb, tv, m.plan.ntiles, m.nrows, ep);
         break;
     case 8:
         hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 8, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
-                           tb, tv, m.plan.ntiles, m.nrows);
+                           tb, tv, m.plan.ntiles, m.nrows, ep);
         break;
     default:
         set_error("invalid r");
@@ -186,12 +187,12 @@ const char* last_error_cstr() {
     return copy.c_str();
 }
 
-int launch_spmv(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s) {
+int launch_spmv(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s, const Epilogue& ep) {
     if (m.plan.ntiles == 0) return SPC5_HIP_OK;
-    if (m.value_type == SPC5_HIP_F64) return launch_spmv_vs<double, double, double, 8, 1>(m, x, y, s);
-    if (m.value_type == SPC5_HIP_F32 && m.vs == 16) return launch_spmv_vs<float, float, float, 16, 2>(m, x, y, s);
-    if (m.value_type == SPC5_HIP_F32 && m.vs == 8) return launch_spmv_vs<float, float, float, 8, 1>(m, x, y, s);
-    if (m.value_type == SPC5_HIP_BF16) return launch_spmv_vs<unsigned short, float, float, 8, 1>(m, x, y, s);
+    if (m.value_type == SPC5_HIP_F64) return launch_spmv_vs<double, double, double, 8, 1>(m, x, y, s, ep);
+    if (m.value_type == SPC5_HIP_F32 && m.vs == 16) return launch_spmv_vs<float, float, float, 16, 2>(m, x, y, s, ep);
+    if (m.value_type == SPC5_HIP_F32 && m.vs == 8) return launch_spmv_vs<float, float, float, 8, 1>(m, x, y, s, ep);
+    if (m.value_type == SPC5_HIP_BF16) return launch_spmv_vs<unsigned short, float, float, 8, 1>(m, x, y, s, ep);
     set_error("unsupported value type / VS");
     return SPC5_HIP_ERR_INVALID;
 }

@@ -141,7 +141,7 @@ template <class V, class X, class ACC, int R, int VS, int MB>
 __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __restrict__ bptr,
                                           const unsigned char* __restrict__ hdr, const V* __restrict__ val,
                                           const X* __restrict__ x, X* __restrict__ y, int br0, int br1, int vstart,
-                                          int64_t nrows) {
+                                          int64_t nrows, const Epilogue& ep) {
     constexpr int K = kExpandCap / 64;
     constexpr int NW = MaskWords<R * MB>::kWords;
     const int lane = lane_id();
@@ -230,7 +230,21 @@ __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __res
     }
 #endif
     wave_sync();
-    for (int i = lane; i < nrow_t; i += 64) y[row0 + i] = (X)((ACC)y[row0 + i] + sm.acc[i]);
+    // y = alpha * (A x) + beta * y; y is not read when beta == 0.  With a dot
+    // vector, the tile also adds sum(dv[i] * y_new[i]) to *ep.dot.
+    const ACC alpha = (ACC)ep.alpha, beta = (ACC)ep.beta;
+    const X* __restrict__ dv = reinterpret_cast<const X*>(ep.dv);
+    double part = 0.0;
+    for (int i = lane; i < nrow_t; i += 64) {
+        const ACC ax = sm.acc[i];
+        const X yn = ep.beta == 0.0 ? (X)(alpha * ax) : (X)(alpha * ax + beta * (ACC)y[row0 + i]);
+        y[row0 + i] = yn;
+        if (dv) part += (double)dv[row0 + i] * (double)yn;
+    }
+    if (ep.dot) {
+        part = wave_sum_f64(part);
+        if (lane == 0) atomicAdd(ep.dot, part);
+    }
 }
 
 }  // namespace spc5

@@ -1,0 +1,58 @@
+"""Batching form: uneven matrices with different r in one persistent
+hipGraph launch.  Bar: every y_i within 1e-12 (double) / 1e-5 (float)
+normwise of the oracle's beta(r,VS) product; several graph replays with
+eager work and synchronisation between them, each checked."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from conftest import normwise_error
+from matgen import clustered_csr, random_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 1e-5)])
+def test_batch_uneven_matrices(dtype, tol):
+    from spc5_amd.batch import Batch
+    from spc5_amd.device import DeviceMatrix
+    rng = np.random.default_rng(0)
+    mats, xs, ys, refs = [], [], [], []
+    for i in range(40):
+        n = int(rng.integers(1, 900))
+        nc = int(rng.integers(1, 700))
+        if i % 3 == 0:
+            rp, col, v = random_csr(n, nc, 0.03, seed=i, dtype=dtype)
+        else:
+            rp, col, v = clustered_csr(n, nc, seed=i, dtype=dtype, empty_frac=0.1)
+        r = (1, 2, 4, 8)[i % 4]
+        m = DeviceMatrix.from_csr(dev(rp), dev(col), dev(v), r, ncols=nc)
+        x = rng.uniform(-1, 1, nc).astype(dtype)
+        y0 = rng.uniform(-1, 1, n).astype(dtype)
+        mats.append(m)
+        xs.append(dev(x))
+        ys.append(dev(y0))
+        vs = O.VS_OF[np.dtype(dtype)]
+        bptr, hdr, vout, nb = O.convert(r, rp, col, v)
+        xp = np.zeros(nc + vs, dtype)
+        xp[:nc] = x
+        yr = np.zeros(n + 8, dtype)
+        if nb:
+            O.beta_spmv(r, n, bptr, hdr, vout, xp, yr)
+        refs.append((rp, v, x, y0, yr[:n].astype(np.float64)))
+    b = Batch(mats, xs, ys)
+    assert b.info()["ntiles"] >= 40
+    for rep in range(3):
+        b.spmv(graph=(rep != 1))
+        torch.cuda.synchronize()
+        for (rp, v, x, y0, yr), y in zip(refs, ys):
+            want = y0.astype(np.float64) + (rep + 1) * yr
+            assert normwise_error(y.cpu().numpy(), want, rp, v * (rep + 1), x, y0) <= tol
+        # eager work between replays
+        torch.ones(1000, device="cuda").sum().item()
+    b.close()

@@ -1,0 +1,139 @@
+"""Precision form on the GPU: reduced-precision value copies, the general
+SpMV epilogue, CG and mixed-precision IR-CG.  No reference counterpart;
+pinned by numpy float64:
+
+* value rounding: bit-identical to torch's double->float and
+  double->bfloat16 conversions (both round to nearest even through float);
+* float / bfloat16 SpMV: within 1e-5 (normwise, float accumulation) of the
+  float64 product with the rounded values, and within 2^-8 of the exact
+  double matrix for bfloat16 (value rounding bound);
+* y = alpha A x + beta y (double): within 1e-12 normwise of numpy;
+* CG / IR-CG: true relative residual, recomputed in numpy float64, below
+  1e-10, and the solution within 1e-7 of numpy.linalg.solve.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import normwise_error
+from matgen import clustered_csr, laplacian2d
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def csr_dense(rp, col, val, ncols):
+    n = len(rp) - 1
+    A = np.zeros((n, ncols))
+    for i in range(n):
+        A[i, col[rp[i]:rp[i + 1]]] = val[rp[i]:rp[i + 1]]
+    return A
+
+
+def stencil27_small(n=12, diag=26.001):
+    from spc5_amd import synth
+    return synth.stencil27(n, diag=diag)
+
+
+@pytest.mark.parametrize("r", (1, 2, 4, 8))
+def test_lowp_values_and_spmv(r):
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import lowp
+    rp, col, val = clustered_csr(700, 900, seed=r, empty_frac=0.05)
+    A64 = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val), r, ncols=900)
+    v64 = A64.values.cpu()
+    x = np.random.default_rng(r).uniform(-1, 1, 900)
+    Ad = csr_dense(rp, col, val, 900)
+    for kind, dt, tol_exact in (("f32", torch.float32, 1e-6), ("bf16", torch.bfloat16, 2.0 ** -8)):
+        Alo = lowp(A64, kind)
+        assert Alo.values.dtype == dt
+        ref_bits = v64.to(dt)
+        assert torch.equal(Alo.values.cpu().view(torch.int16 if dt == torch.bfloat16 else torch.int32),
+                           ref_bits.view(torch.int16 if dt == torch.bfloat16 else torch.int32))
+        # the rounded matrix in float64, same permutation as the stream
+        rounded = np.zeros_like(val)
+        order_vals = ref_bits.to(torch.float64).numpy()
+        # decode permuted order back to CSR order through the oracle
+        import oracle as O
+        bptr = A64.rows_size.cpu().numpy()
+        rows, cols, _ = O.decode(r, 700, bptr, A64.headers.cpu().numpy(), v64.numpy())
+        Alo_d = np.zeros((700, 900))
+        Alo_d[rows, cols] = order_vals
+        xf = x.astype(np.float32)
+        y = torch.zeros(700, dtype=torch.float32, device="cuda")
+        Alo.spmv(dev(xf), y)
+        yref = Alo_d @ xf.astype(np.float64)
+        assert normwise_error(y.cpu().numpy(), yref, rp, val, xf, np.zeros(1)) <= 1e-5
+        assert normwise_error(y.cpu().numpy(), Ad @ xf.astype(np.float64), rp, val, xf, np.zeros(1)) <= tol_exact
+        del rounded
+
+
+@pytest.mark.parametrize("alpha,beta", [(1.0, 1.0), (1.0, 0.0), (-1.0, 1.0), (2.5, -0.5)])
+def test_spmv_ex_double(alpha, beta):
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import spmv_ex
+    rp, col, val = clustered_csr(500, 500, seed=7)
+    A = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val), 4)
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1, 1, 500)
+    y0 = rng.uniform(-1, 1, 500)
+    y = dev(y0.copy())
+    spmv_ex(A, dev(x), y, alpha, beta)
+    yref = alpha * (csr_dense(rp, col, val, 500) @ x) + beta * y0
+    assert normwise_error(y.cpu().numpy(), yref, rp, val * max(1, abs(alpha)), x, y0) <= 1e-12
+
+
+@pytest.mark.parametrize("r", (1, 4, 8))
+def test_cg_double(r):
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import cg
+    rp, col, val = laplacian2d(24)
+    n = len(rp) - 1
+    A = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val), r)
+    b = np.random.default_rng(r).uniform(-1, 1, n)
+    x, st = cg(A, dev(b), rtol=1e-10, maxit=5000)
+    xh = x.cpu().numpy()
+    Ad = csr_dense(rp, col, val, n)
+    relres = np.linalg.norm(b - Ad @ xh) / np.linalg.norm(b)
+    assert relres <= 1e-10, (relres, st)
+    assert st["relres"] == pytest.approx(relres, rel=1e-3, abs=1e-14)
+    xs = np.linalg.solve(Ad, b)
+    assert np.linalg.norm(xh - xs) / np.linalg.norm(xs) <= 1e-7
+
+
+@pytest.mark.parametrize("kind", ["f32", "bf16"])
+@pytest.mark.parametrize("r", (2, 4))
+def test_ir_cg_mixed(kind, r):
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import ir_cg, lowp
+    rp, col, val = stencil27_small(12)
+    n = len(rp) - 1
+    A64 = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val), r)
+    Alo = lowp(A64, kind)
+    b = np.random.default_rng(5).uniform(-1, 1, n)
+    x, st = ir_cg(A64, Alo, dev(b), rtol=1e-10)
+    xh = x.cpu().numpy()
+    Ad = csr_dense(rp, col, val, n)
+    relres = np.linalg.norm(b - Ad @ xh) / np.linalg.norm(b)
+    assert relres <= 1e-10, (relres, st)
+    assert st["outer"] >= 1
+    xs = np.linalg.solve(Ad, b)
+    assert np.linalg.norm(xh - xs) / np.linalg.norm(xs) <= 1e-7
+
+
+def test_cg_float_reference_format():
+    """Float CG on the reference float format (VS=16): converges to 1e-5."""
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import cg
+    rp, col, val = laplacian2d(16)
+    n = len(rp) - 1
+    A = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val.astype(np.float32)), 2)
+    assert A.vs == 16
+    b = np.random.default_rng(2).uniform(-1, 1, n).astype(np.float32)
+    x, st = cg(A, dev(b), rtol=1e-5, maxit=2000)
+    Ad = csr_dense(rp, col, val, n)
+    xh = x.cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(b - Ad @ xh) / np.linalg.norm(b) <= 2e-5
