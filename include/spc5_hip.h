@@ -194,6 +194,16 @@ int spc5_hip_ir_cg(spc5_hip_mat_t A64, spc5_hip_mat_t Alo, const double* d_b, do
 /* (no counterpart in the reference).                                     */
 /* ---------------------------------------------------------------------- */
 
+/* Y = alpha * (A X) + beta * Y for k right-hand sides stored as row-major
+ * dense blocks: X is ncols x k with leading dimension ldx, Y is nrows x k
+ * with leading dimension ldy (device pointers, the matrix's vector type).
+ * kt is the column chunk one pass handles (0 or >= k: all k columns in
+ * one pass, the matrix read once); smaller chunks re-read the matrix k/kt
+ * times with a smaller x working set.  Tile lists for the chunk width are
+ * built on first use and kept with the handle. */
+int spc5_hip_spmm(spc5_hip_mat_t h, int k, int kt, double alpha, const void* d_x, int64_t ldx, double beta,
+                  void* d_y, int64_t ldy, void* stream);
+
 typedef struct spc5_hip_batch* spc5_hip_batch_t;
 
 /* Group nmats matrix handles (one value type and VS; each its own r) with

@@ -72,6 +72,11 @@ namespace spc5 {
 int launch_spmv(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s, const Epilogue& ep = Epilogue());
 // Build the tile plan of a handle (defined in spmv.hip).
 int build_plan(spc5_hip_mat& m, hipStream_t s);
+// Another tile list of the same matrix with at most `maxrows` rows per tile
+// (needs the value offsets of m.plan; defined in spmv.hip).
+int build_tiles(const spc5_hip_mat& m, Plan& p, int maxrows, int target, hipStream_t s);
+// Drops the multi-right-hand-side tile lists cached for a handle (spmm.hip).
+void forget_spmm_plans(const spc5_hip_mat* m);
 
 inline bool valid_format(int vt, int vs, int mb) {
     return (vt == SPC5_HIP_F64 && vs == 8 && mb == 1) || (vt == SPC5_HIP_F32 && vs == 16 && mb == 2) ||

@@ -250,12 +250,25 @@ int build_plan(spc5_hip_mat& m, hipStream_t s) {
         set_error("nnz (" + std::to_string(m.nnz) + ") differs from the mask popcount (" + std::to_string(hnnz) + ")");
         return SPC5_HIP_ERR_INVALID;
     }
-    // Tile cut: value target per tile, and at most kMaxRows rows per tile.
     int64_t target = m.nnz / 8192;
     if (target < 512) target = 512;
     if (target > 8192) target = 8192;
-    p.target = (int)target;
-    const int maxbr = kMaxRows / m.r;
+    return build_tiles(m, p, kMaxRows, (int)target, s);
+}
+
+// Tile cut from the block-row value offsets: a tile starts where the running
+// value count crosses a multiple of `target`, and at least every
+// maxrows / r block-rows.
+int build_tiles(const spc5_hip_mat& m, Plan& p, int maxrows, int target, hipStream_t s) {
+    p.ntiles = 0;
+    p.target = target;
+    if (m.nbr == 0) return SPC5_HIP_OK;
+    if (!p.valptr) p.valptr = m.plan.valptr;
+    const int* valptr = p.valptr->as<int>();
+    const int maxbr = maxrows / m.r > 0 ? maxrows / m.r : 1;
+    int rc;
+    DeviceBuffer dtotal;
+    SPC5_HIP_TRY(dtotal.alloc(sizeof(int)));
     DeviceBuffer flags;
     SPC5_HIP_TRY(flags.alloc(sizeof(int) * (size_t)(m.nbr + 1)));
     const unsigned g = (unsigned)((m.nbr + 1 + 255) / 256);
@@ -333,6 +346,7 @@ int spc5_hip_mat_create_values_view(spc5_hip_mat_t base, int value_type, const v
 }
 
 int spc5_hip_mat_destroy(spc5_hip_mat_t h) {
+    if (h) forget_spmm_plans(h);
     delete h;
     return SPC5_HIP_OK;
 }

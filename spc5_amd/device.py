@@ -160,6 +160,21 @@ class DeviceMatrix:
         check(lib().spc5_hip_spmv(self._h, _ptr(x), _ptr(y), stream_ptr(stream)), "spc5_hip_spmv")
         return y
 
+    def spmm(self, X, Y, alpha=1.0, beta=1.0, kt: int = 0, stream=None):
+        """Y = alpha * (A X) + beta * Y for row-major blocks X (ncols x k) and
+        Y (nrows x k); kt = columns per pass (0: all k, one read of A)."""
+        for t, nm, rows in ((X, "X", self.ncols), (Y, "Y", self.nrows)):
+            if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dim() != 2 or t.dtype != self.vec_dtype:
+                raise Spc5Error(f"{nm} must be a 2-D {self.vec_dtype} CUDA tensor")
+            if t.stride(1) != 1 or t.shape[0] < rows:
+                raise Spc5Error(f"{nm} must be row-major with >= {rows} rows")
+        k = X.shape[1]
+        if Y.shape[1] != k:
+            raise Spc5Error("X and Y need the same number of columns")
+        check(lib().spc5_hip_spmm(self._h, k, int(kt), float(alpha), _ptr(X), X.stride(0), float(beta), _ptr(Y),
+                                  Y.stride(0), stream_ptr(stream)), "spc5_hip_spmm")
+        return Y
+
     def close(self):
         if self._h and self._h.value:
             lib().spc5_hip_mat_destroy(self._h)
