@@ -191,6 +191,111 @@ def solve_benchmark(n=160, r=4, rtol=1e-10):
     return out
 
 
+def _events_time(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps * 1e-3
+
+
+def spmm_benchmark(n=160, ks=(1, 4, 16, 64), steps=10, warmup=2):
+    """Multi-right-hand-side products on the 27-point stencil n^3, double and
+    float values (beta(r,8)), row-major X and Y with k columns; the column
+    chunk kt is picked by the autotuner (timed candidates)."""
+    import torch
+    from spc5_amd import autotune, synth
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import lowp
+
+    rp, col, val = synth.stencil27(n)
+    N = len(rp) - 1
+    nnz = len(val)
+    d_rp, d_col, d_val = (torch.from_numpy(a).cuda() for a in (rp, col, val))
+    r, det = autotune.choose_r(d_rp, d_col, d_val, mode="model")
+    A64 = DeviceMatrix.from_csr(d_rp, d_col, d_val, r)
+    del d_rp, d_col, d_val
+    out = {"workload": f"stencil27 {n}^3 (N={N}, nnz={nnz}), beta({r},8) chosen by the autotuner",
+           "blocks": {str(k): v for k, v in det["blocks"].items()}, "cases": []}
+    g = torch.Generator(device="cuda").manual_seed(3)
+    for tag, A in (("f64", A64), ("f32", lowp(A64, "f32"))):
+        for k in ks:
+            X = torch.empty((N, k), dtype=A.vec_dtype, device="cuda").uniform_(-1, 1, generator=g)
+            Y = torch.zeros((N, k), dtype=A.vec_dtype, device="cuda")
+            kt, cand = autotune.tune_spmm_kt(A, k)
+            t = _events_time(lambda: A.spmm(X, Y, kt=kt), steps, warmup)
+            a_bytes = A.algorithmic_bytes() - (A.ncols + 2 * A.nrows) * A.vec_dtype.itemsize
+            B = a_bytes + k * (A.ncols + 2 * A.nrows) * A.vec_dtype.itemsize
+            case = {"dtype": tag, "k": k, "kt": kt, "ms": round(t * 1e3, 4),
+                    "gflops": round(2.0 * nnz * k / t / 1e9, 1), "gbs": round(B / t / 1e9, 1),
+                    "frac_of_peak": round(B / t / 1e9 / HBM_PEAK_GBS, 4), "bytes": B,
+                    "kt_candidates_ms": {str(c): round(v * 1e3, 4) for c, v in cand.items()}}
+            out["cases"].append(case)
+            log(f"[bench] SpMM {tag} k={k} kt={kt}: {t * 1e3:.3f} ms {case['gflops']} GFLOP/s {case['gbs']} GB/s")
+            del X, Y
+    del A64
+    torch.cuda.empty_cache()
+    return out
+
+
+def batch_benchmark(count=2000, fill=0.3, steps=10, warmup=2):
+    """The uneven batch: `count` banded matrices (rows log-uniform in
+    [1e3, 2e5], bandwidth uniform in {4..256}, 30% fill inside the band,
+    diagonal + bandwidth), generated on the GPU, r per matrix from the
+    autotuner's model, all products in one persistent graph launch; double
+    and float (beta(r,8) with float values)."""
+    import torch
+    from spc5_amd import autotune, synth
+    from spc5_amd.batch import Batch
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import lowp
+
+    t0 = time.perf_counter()
+    params = synth.uneven_batch_params(count)
+    mats, rs, nnz, rows = [], {1: 0, 2: 0, 4: 0, 8: 0}, 0, 0
+    for (n, bw, seed) in params:
+        rp, col, val = synth.banded_gpu(n, bw, fill, seed)
+        r, _ = autotune.choose_r(rp, col, val, mode="model")
+        mats.append(DeviceMatrix.from_csr(rp, col, val, r))
+        rs[r] += 1
+        nnz += val.numel()
+        rows += n
+        del rp, col, val
+    setup = time.perf_counter() - t0
+    log(f"[bench] batch: {count} matrices, {rows} rows, {nnz} values, r histogram {rs}, setup {setup:.1f}s")
+    out = {"workload": f"{count} banded matrices, rows log-uniform [1e3,2e5], bw U{{4..256}}, fill {fill}",
+           "matrices": count, "rows": rows, "nnz": nnz, "r_histogram": {str(k): v for k, v in rs.items()},
+           "setup_s": round(setup, 1), "cases": []}
+    g = torch.Generator(device="cuda").manual_seed(5)
+    for tag in ("f64", "f32"):
+        ms = mats if tag == "f64" else [lowp(m, "f32") for m in mats]
+        xs = [torch.empty(m.ncols, dtype=m.vec_dtype, device="cuda").uniform_(-1, 1, generator=g) for m in ms]
+        ys = [torch.zeros(m.nrows, dtype=m.vec_dtype, device="cuda") for m in ms]
+        b = Batch(ms, xs, ys)
+        t = _events_time(lambda: b.spmv(graph=True), steps, warmup)
+        B = b.algorithmic_bytes()
+        info = b.info()
+        case = {"dtype": tag, "ms": round(t * 1e3, 3), "gflops": round(2.0 * nnz / t / 1e9, 1),
+                "gbs": round(B / t / 1e9, 1), "frac_of_peak": round(B / t / 1e9 / HBM_PEAK_GBS, 4),
+                "bytes": B, "tiles": info["ntiles"], "persistent_waves": info["waves"], "launch": "hipGraph"}
+        out["cases"].append(case)
+        log(f"[bench] batch {tag}: {t * 1e3:.2f} ms {case['gflops']} GFLOP/s {case['gbs']} GB/s")
+        b.close()
+        del b, xs, ys
+        if tag == "f32":
+            del ms
+    del mats
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +310,10 @@ def main():
     ap.add_argument("--no-copy", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--solve-n", type=int, default=160)
+    ap.add_argument("--no-spmm", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=2000)
+    ap.add_argument("--only-extensions", action="store_true", help="skip the headline SpMV sections")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -281,6 +390,10 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_solve:
         out["solve"] = solve_benchmark(args.solve_n)
+    if not args.no_spmm:
+        out["spmm"] = spmm_benchmark(args.solve_n)
+    if not args.no_batch:
+        out["batch"] = batch_benchmark(args.batch_count)
     if not args.no_cpu:
         log("[bench] CPU baseline (reference OpenMP path)...")
         th = args.cpu_threads or os.cpu_count() or 1

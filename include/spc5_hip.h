@@ -224,6 +224,22 @@ int spc5_hip_batch_info(spc5_hip_batch_t b, int64_t* ntiles, int* waves);
 int spc5_hip_batch_destroy(spc5_hip_batch_t b);
 
 /* ---------------------------------------------------------------------- */
+/* Benchmark workloads generated on the device (counterpart of the        */
+/* reference driver's synthetic builddense,                               */
+/* examples/load_mm_and_compare.cpp:431-449).                             */
+/* ---------------------------------------------------------------------- */
+
+/* Banded n x n CSR matrix, byte-identical to spc5_synth_banded_count/fill
+ * of libspc5_synth.so: entry (i,j), |i-j| <= bw, kept on the diagonal and
+ * where splitmix64(seed*0x100000001b3 ^ (i<<24 ^ j)) < thr; values U[-1,1]
+ * (splitmix64 on seed+1 and the CSR position), diagonal + bw.  count writes
+ * d_rowptr[0..n] and *nnz; fill writes the columns and values.  n < 2^24. */
+int spc5_hip_synth_banded_count(int64_t n, int64_t bw, uint64_t thr, uint64_t seed, int* d_rowptr, int64_t* nnz,
+                                void* stream);
+int spc5_hip_synth_banded_fill(int64_t n, int64_t bw, uint64_t thr, uint64_t seed, const int* d_rowptr, int* d_col,
+                               double* d_val, void* stream);
+
+/* ---------------------------------------------------------------------- */
 /* 3. Host-pointer helpers (used by include/spc5_hip.hpp)                 */
 /* ---------------------------------------------------------------------- */
 

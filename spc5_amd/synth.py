@@ -96,6 +96,44 @@ def clustered_runs(N: int, seed: int = 1, value_seed: int = 2):
     return rowptr, col, val
 
 
+def banded_threshold(fill: float) -> int:
+    """Keep-threshold of the banded generators: floor(fill * 2^64), saturated."""
+    t = fill * 18446744073709551616.0
+    return 0xFFFFFFFFFFFFFFFF if t >= 18446744073709551615.0 else int(t)
+
+
+def banded_gpu(n: int, bw: int, fill: float, seed: int):
+    """The banded matrix of `banded`, generated on the GPU (device tensors
+    rowptr int32, col int32, val float64); byte-identical to the host one."""
+    import torch
+    from ._lib import check
+    from ._lib import lib as hip
+    thr = banded_threshold(fill)
+    rowptr = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+    nnz = C.c_int64(0)
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check(hip().spc5_hip_synth_banded_count(n, bw, thr, seed, C.c_void_p(rowptr.data_ptr()), C.byref(nnz), s),
+          "spc5_hip_synth_banded_count")
+    col = torch.empty(max(1, nnz.value), dtype=torch.int32, device="cuda")
+    val = torch.empty(max(1, nnz.value), dtype=torch.float64, device="cuda")
+    check(hip().spc5_hip_synth_banded_fill(n, bw, thr, seed, C.c_void_p(rowptr.data_ptr()),
+                                           C.c_void_p(col.data_ptr()), C.c_void_p(val.data_ptr()), s),
+          "spc5_hip_synth_banded_fill")
+    return rowptr, col[: nnz.value], val[: nnz.value]
+
+
+def uneven_batch_params(count: int = 2000, seed: int = 11):
+    """Per-matrix (rows, bandwidth, seed) of the uneven batch: rows
+    log-uniform in [1,000, 200,000], bandwidth uniform in {4..256}."""
+    u = uniform(2 * count, seed, 0.0, 1.0)
+    out = []
+    for i in range(count):
+        rows = int(round(1000.0 * (200.0 ** u[2 * i])))
+        bw = 4 + int(u[2 * i + 1] * 253.0)
+        out.append((rows, min(bw, 256), 1000 + i))
+    return out
+
+
 def banded(n: int, bw: int, fill: float, seed: int):
     rowptr = np.empty(n + 1, np.int32)
     nnz = lib().spc5_synth_banded_count(n, bw, fill, seed, _p(rowptr))

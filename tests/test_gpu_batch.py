@@ -56,3 +56,15 @@ def test_batch_uneven_matrices(dtype, tol):
         # eager work between replays
         torch.ones(1000, device="cuda").sum().item()
     b.close()
+
+
+@pytest.mark.parametrize("n,bw,fill,seed", [(1000, 4, 0.3, 7), (5003, 256, 0.3, 8), (777, 31, 0.9, 9)])
+def test_banded_generator_gpu_equals_host(n, bw, fill, seed):
+    """The device generator of the uneven batch is byte-identical to the
+    host generator (same hashing and value formula)."""
+    from spc5_amd import synth
+    rp, col, val = synth.banded(n, bw, fill, seed)
+    drp, dcol, dval = synth.banded_gpu(n, bw, fill, seed)
+    assert np.array_equal(drp.cpu().numpy(), rp)
+    assert np.array_equal(dcol.cpu().numpy(), col)
+    assert np.array_equal(dval.cpu().numpy().view(np.uint64), val.view(np.uint64))
