@@ -25,6 +25,7 @@ struct BatchEntry {
     const int* tile_br;
     const int* tile_val;
     int64_t nrows;
+    int64_t ncols;
     int r;
     int pad;
 };
@@ -52,12 +53,12 @@ struct spc5_hip_batch {
 namespace spc5 {
 namespace {
 
-template <class ACC>
+template <class X, class ACC, int VS>
 union BatchSmem {
-    TileSmem<ACC, 1> r1;
-    TileSmem<ACC, 2> r2;
-    TileSmem<ACC, 4> r4;
-    TileSmem<ACC, 8> r8;
+    StreamSmem<X, ACC, 1, VS> r1;
+    StreamSmem<X, ACC, 2, VS> r2;
+    StreamSmem<X, ACC, 4, VS> r4;
+    StreamSmem<X, ACC, 8, VS> r8;
 };
 
 template <class V, class X, class ACC, int VS, int MB>
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_batch(const BatchE
                                                                      const int* __restrict__ tile_mat,
                                                                      const int* __restrict__ tile_loc,
                                                                      int64_t ntiles) {
-    __shared__ BatchSmem<ACC> sm[kWavesPerGroup];
+    __shared__ BatchSmem<X, ACC, VS> sm[kWavesPerGroup];
     const int w = threadIdx.x >> 6;
     const int64_t stride = (int64_t)gridDim.x * kWavesPerGroup;
     const Epilogue ep;
@@ -77,10 +78,22 @@ __global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_batch(const BatchE
         const X* x = reinterpret_cast<const X*>(m.x);
         X* y = reinterpret_cast<X*>(m.y);
         switch (m.r) {
-        case 1: spmv_tile<V, X, ACC, 1, VS, MB>(sm[w].r1, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, ep); break;
-        case 2: spmv_tile<V, X, ACC, 2, VS, MB>(sm[w].r2, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, ep); break;
-        case 4: spmv_tile<V, X, ACC, 4, VS, MB>(sm[w].r4, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, ep); break;
-        default: spmv_tile<V, X, ACC, 8, VS, MB>(sm[w].r8, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, ep); break;
+        case 1:
+            spmv_tile_stream<V, X, ACC, 1, VS, MB>(sm[w].r1, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, m.ncols,
+                                                   ep);
+            break;
+        case 2:
+            spmv_tile_stream<V, X, ACC, 2, VS, MB>(sm[w].r2, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, m.ncols,
+                                                   ep);
+            break;
+        case 4:
+            spmv_tile_stream<V, X, ACC, 4, VS, MB>(sm[w].r4, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, m.ncols,
+                                                   ep);
+            break;
+        default:
+            spmv_tile_stream<V, X, ACC, 8, VS, MB>(sm[w].r8, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, m.ncols,
+                                                   ep);
+            break;
         }
         wave_sync();
     }
@@ -137,6 +150,7 @@ int spc5_hip_batch_create(int nmats, const spc5_hip_mat_t* mats, const void* con
         e.tile_br = m->plan.ntiles ? m->plan.tile_br->as<const int>() : nullptr;
         e.tile_val = m->plan.ntiles ? m->plan.tile_val->as<const int>() : nullptr;
         e.nrows = m->nrows;
+        e.ncols = m->ncols;
         e.r = m->r;
         e.pad = 0;
         for (int64_t t = 0; t < m->plan.ntiles; ++t) {

@@ -2,12 +2,15 @@
 // on beta(r,VS) matrices (the precision form; no counterpart in the
 // reference).
 //
-// CG iteration, three kernels, scalars kept on the device:
-//   K1  q = A p            (SpMV, y = A x epilogue, fused pq += p.q)
-//   K2  r -= alpha q       (alpha = rr / pq), rr_new += r.r
-//   K3  x += alpha p, p = r + beta p   (beta = rr_new / rr); the last
-//       workgroup to finish rotates rr <- rr_new and clears pq, rr_new.
-// The host reads rr every `check` iterations to stop.
+// CG iteration, scalars kept on the device (no host round trip):
+//   K1  q = A p            (SpMV, y = A x epilogue; each tile writes its
+//                           partial of p.q) -> pq = sum of the partials
+//   K2  r -= alpha q       (alpha = rr / pq); per-workgroup partials of r.r
+//                           -> rr_new
+//   K3  x += alpha p, p = r + beta p   (beta = rr_new / rr), then rr = rr_new
+// Reductions never use global atomics: partials are summed in a fixed order
+// by one workgroup, so a solve is bit-reproducible.  The host reads rr every
+// `check` iterations to stop.
 //
 // Iterative refinement (IR-CG): x64 = 0, r64 = b; repeat
 //   r32 = r64 / ||r64||  (float);  solve A_lo d = r32 with float CG to a
@@ -16,6 +19,7 @@
 //   (double SpMV) -- until ||r64|| <= rtol ||b||.
 #include "spc5_mat.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -29,10 +33,10 @@ struct CgScalars {
     double rr_new;
     double pq;
     double pad;
-    unsigned int done;
 };
 
 constexpr int kVecThreads = 256;
+constexpr int kMaxVecGroups = 2048;
 
 __device__ __forceinline__ double block_sum(double v, double* s_part) {
     v = wave_sum_f64(v);
@@ -45,15 +49,25 @@ __device__ __forceinline__ double block_sum(double v, double* s_part) {
     return t;  // valid in thread 0
 }
 
+// out = sum of part[0..n), fixed order (one workgroup).
+__global__ __launch_bounds__(kVecThreads) void k_reduce_sum(const double* __restrict__ part, int64_t n,
+                                                            double* __restrict__ out) {
+    __shared__ double s_part[kVecThreads / 64];
+    double p = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += kVecThreads) p += part[i];
+    p = block_sum(p, s_part);
+    if (threadIdx.x == 0) *out = p;
+}
+
 template <class X>
 __global__ __launch_bounds__(kVecThreads) void k_dot(const X* __restrict__ a, const X* __restrict__ b, int64_t n,
-                                                     double* out) {
+                                                     double* __restrict__ part) {
     __shared__ double s_part[kVecThreads / 64];
     double p = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads)
         p += (double)a[i] * (double)b[i];
     p = block_sum(p, s_part);
-    if (threadIdx.x == 0) atomicAdd(out, p);
+    if (threadIdx.x == 0) part[blockIdx.x] = p;
 }
 
 template <class X>
@@ -84,10 +98,10 @@ __global__ __launch_bounds__(kVecThreads) void k_add_scaled_f32(double* __restri
         x[i] += s * (double)d[i];
 }
 
-// K2: r -= alpha q; rr_new += r.r
+// K2: r -= alpha q; partials of r.r
 template <class X>
 __global__ __launch_bounds__(kVecThreads) void k_cg_r(X* __restrict__ r, const X* __restrict__ q, int64_t n,
-                                                      CgScalars* sc) {
+                                                      const CgScalars* __restrict__ sc, double* __restrict__ part) {
     __shared__ double s_part[kVecThreads / 64];
     const X alpha = (X)(sc->rr / sc->pq);
     double p = 0.0;
@@ -97,14 +111,13 @@ __global__ __launch_bounds__(kVecThreads) void k_cg_r(X* __restrict__ r, const X
         p += (double)ri * (double)ri;
     }
     p = block_sum(p, s_part);
-    if (threadIdx.x == 0) atomicAdd(&sc->rr_new, p);
+    if (threadIdx.x == 0) part[blockIdx.x] = p;
 }
 
-// K3: x += alpha p; p = r + beta p; the last workgroup rotates the scalars.
+// K3: x += alpha p; p = r + beta p
 template <class X>
 __global__ __launch_bounds__(kVecThreads) void k_cg_xp(X* __restrict__ x, X* __restrict__ p, const X* __restrict__ r,
-                                                       int64_t n, CgScalars* sc) {
-    __shared__ bool s_last;
+                                                       int64_t n, const CgScalars* __restrict__ sc) {
     const double rr = sc->rr;
     const X alpha = (X)(rr / sc->pq);
     const X beta = (X)(sc->rr_new / rr);
@@ -113,37 +126,30 @@ __global__ __launch_bounds__(kVecThreads) void k_cg_xp(X* __restrict__ x, X* __r
         x[i] += alpha * pi;
         p[i] = r[i] + beta * pi;
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicInc(&sc->done, gridDim.x - 1) == gridDim.x - 1;
-    __syncthreads();
-    if (s_last && threadIdx.x == 0) {
-        __threadfence();
-        sc->rr = sc->rr_new;
-        sc->rr_new = 0.0;
-        sc->pq = 0.0;
-    }
 }
+
+__global__ void k_cg_rotate(CgScalars* sc) { sc->rr = sc->rr_new; }
 
 __global__ void k_cg_init_scalars(CgScalars* sc, const double* rr) {
     sc->rr = *rr;
     sc->rr_new = 0.0;
     sc->pq = 0.0;
     sc->pad = 0.0;
-    sc->done = 0u;
 }
 
 unsigned vec_grid(int64_t n) {
     int64_t g = (n + kVecThreads - 1) / kVecThreads;
-    if (g > 8192) g = 8192;
+    if (g > kMaxVecGroups) g = kMaxVecGroups;
     if (g < 1) g = 1;
     return (unsigned)g;
 }
 
+// *d_out = a.b (partials in `part`, at least vec_grid(n) doubles)
 template <class X>
-int dot(const X* a, const X* b, int64_t n, double* d_out, hipStream_t s) {
-    SPC5_HIP_TRY(hipMemsetAsync(d_out, 0, sizeof(double), s));
-    hipLaunchKernelGGL(k_dot<X>, dim3(vec_grid(n)), dim3(kVecThreads), 0, s, a, b, n, d_out);
+int dot(const X* a, const X* b, int64_t n, double* part, double* d_out, hipStream_t s) {
+    const unsigned g = vec_grid(n);
+    hipLaunchKernelGGL(k_dot<X>, dim3(g), dim3(kVecThreads), 0, s, a, b, n, part);
+    hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kVecThreads), 0, s, part, (int64_t)g, d_out);
     SPC5_HIP_TRY(hipGetLastError());
     return SPC5_HIP_OK;
 }
@@ -161,16 +167,19 @@ int cg_impl(const spc5_hip_mat& A, const X* b, X* x, double rtol, int maxit, int
             hipStream_t s) {
     const int64_t n = A.nrows;
     SPC5_CHECK_ARG(A.nrows == A.ncols, "CG needs a square matrix");
-    DeviceBuffer vr, vp, vq, vsc, vtmp;
+    const int64_t npart = std::max<int64_t>(kMaxVecGroups, A.plan.ntiles);
+    DeviceBuffer vr, vp, vq, vsc, vtmp, vpart;
     SPC5_HIP_TRY(vr.alloc(sizeof(X) * (size_t)n));
     SPC5_HIP_TRY(vp.alloc(sizeof(X) * (size_t)n));
     SPC5_HIP_TRY(vq.alloc(sizeof(X) * (size_t)n));
     SPC5_HIP_TRY(vsc.alloc(sizeof(CgScalars)));
     SPC5_HIP_TRY(vtmp.alloc(2 * sizeof(double)));
+    SPC5_HIP_TRY(vpart.alloc(sizeof(double) * (size_t)npart));
     X* r = vr.as<X>();
     X* p = vp.as<X>();
     X* q = vq.as<X>();
     CgScalars* sc = vsc.as<CgScalars>();
+    double* part = vpart.as<double>();
     double* d_bb = vtmp.as<double>();
     double* d_rr = d_bb + 1;
     const unsigned g = vec_grid(n);
@@ -182,8 +191,8 @@ int cg_impl(const spc5_hip_mat& A, const X* b, X* x, double rtol, int maxit, int
     ep.beta = 1.0;
     if ((rc = launch_spmv(A, x, r, s, ep))) return rc;
     hipLaunchKernelGGL(k_copy<X>, dim3(g), dim3(kVecThreads), 0, s, p, r, n);
-    if ((rc = dot<X>(b, b, n, d_bb, s))) return rc;
-    if ((rc = dot<X>(r, r, n, d_rr, s))) return rc;
+    if ((rc = dot<X>(b, b, n, part, d_bb, s))) return rc;
+    if ((rc = dot<X>(r, r, n, part, d_rr, s))) return rc;
     hipLaunchKernelGGL(k_cg_init_scalars, dim3(1), dim3(1), 0, s, sc, d_rr);
     SPC5_HIP_TRY(hipGetLastError());
     double hbb = 0.0, hrr = 0.0;
@@ -195,13 +204,16 @@ int cg_impl(const spc5_hip_mat& A, const X* b, X* x, double rtol, int maxit, int
     ek1.alpha = 1.0;
     ek1.beta = 0.0;
     ek1.dv = p;
-    ek1.dot = &sc->pq;
+    ek1.dot = part;
     while (hrr > stop2 && it < maxit) {
         const int chunk = (maxit - it) < check ? (maxit - it) : check;
         for (int c = 0; c < chunk; ++c) {
             if ((rc = launch_spmv(A, p, q, s, ek1))) return rc;
-            hipLaunchKernelGGL(k_cg_r<X>, dim3(g), dim3(kVecThreads), 0, s, r, q, n, sc);
+            hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kVecThreads), 0, s, part, A.plan.ntiles, &sc->pq);
+            hipLaunchKernelGGL(k_cg_r<X>, dim3(g), dim3(kVecThreads), 0, s, r, q, n, sc, part);
+            hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kVecThreads), 0, s, part, (int64_t)g, &sc->rr_new);
             hipLaunchKernelGGL(k_cg_xp<X>, dim3(g), dim3(kVecThreads), 0, s, x, p, r, n, sc);
+            hipLaunchKernelGGL(k_cg_rotate, dim3(1), dim3(1), 0, s, sc);
         }
         SPC5_HIP_TRY(hipGetLastError());
         it += chunk;
@@ -211,7 +223,7 @@ int cg_impl(const spc5_hip_mat& A, const X* b, X* x, double rtol, int maxit, int
     // true residual ||b - A x|| / ||b||
     hipLaunchKernelGGL(k_copy<X>, dim3(g), dim3(kVecThreads), 0, s, r, b, n);
     if ((rc = launch_spmv(A, x, r, s, ep))) return rc;
-    if ((rc = dot<X>(r, r, n, d_rr, s))) return rc;
+    if ((rc = dot<X>(r, r, n, part, d_rr, s))) return rc;
     if ((rc = read_scalar(d_rr, &hrr, s))) return rc;
     if (st) {
         st->iterations = it;
@@ -285,14 +297,16 @@ int spc5_hip_ir_cg(spc5_hip_mat_t A64, spc5_hip_mat_t Alo, const double* d_b, do
     SPC5_CHECK_ARG(rtol > 0.0 && inner_rtol > 0.0 && inner_rtol < 1.0, "tolerances out of range");
     hipStream_t s = as_stream(stream);
     const int64_t n = A64->nrows;
-    DeviceBuffer vr, vr32, vd32, vtmp;
+    DeviceBuffer vr, vr32, vd32, vtmp, vpart;
     SPC5_HIP_TRY(vr.alloc(sizeof(double) * (size_t)n));
     SPC5_HIP_TRY(vr32.alloc(sizeof(float) * (size_t)n));
     SPC5_HIP_TRY(vd32.alloc(sizeof(float) * (size_t)n));
     SPC5_HIP_TRY(vtmp.alloc(2 * sizeof(double)));
+    SPC5_HIP_TRY(vpart.alloc(sizeof(double) * kMaxVecGroups));
     double* r = vr.as<double>();
     float* r32 = vr32.as<float>();
     float* d32 = vd32.as<float>();
+    double* part = vpart.as<double>();
     double* d_bb = vtmp.as<double>();
     double* d_rr = d_bb + 1;
     const unsigned g = vec_grid(n);
@@ -302,13 +316,13 @@ int spc5_hip_ir_cg(spc5_hip_mat_t A64, spc5_hip_mat_t Alo, const double* d_b, do
     ep.beta = 1.0;
     hipLaunchKernelGGL(k_copy<double>, dim3(g), dim3(kVecThreads), 0, s, r, d_b, n);
     if ((rc = launch_spmv(*A64, d_x, r, s, ep))) return rc;
-    if ((rc = dot<double>(d_b, d_b, n, d_bb, s))) return rc;
+    if ((rc = dot<double>(d_b, d_b, n, part, d_bb, s))) return rc;
     double hbb = 0.0, hrr = 0.0;
     if ((rc = read_scalar(d_bb, &hbb, s))) return rc;
     if (hbb == 0.0) hbb = 1.0;
     int outer = 0, inner_total = 0;
     for (;;) {
-        if ((rc = dot<double>(r, r, n, d_rr, s))) return rc;
+        if ((rc = dot<double>(r, r, n, part, d_rr, s))) return rc;
         if ((rc = read_scalar(d_rr, &hrr, s))) return rc;
         if (hrr <= rtol * rtol * hbb || outer >= max_outer || !std::isfinite(hrr)) break;
         hipLaunchKernelGGL(k_scale_to_f32, dim3(g), dim3(kVecThreads), 0, s, r32, r, n, d_rr);

@@ -14,16 +14,19 @@ namespace spc5 {
 // tile's rows fit the wave's LDS accumulators and every tile carries about
 // the same number of values.
 #ifndef SPC5_MAX_ROWS
-#define SPC5_MAX_ROWS 256
+#define SPC5_MAX_ROWS 128
 #endif
 #ifndef SPC5_WAVES_PER_GROUP
-#define SPC5_WAVES_PER_GROUP 4
+#define SPC5_WAVES_PER_GROUP 1
 #endif
 #ifndef SPC5_EXPAND_CAP
 #define SPC5_EXPAND_CAP 512
 #endif
 #ifndef SPC5_SEGREDUCE
 #define SPC5_SEGREDUCE 1
+#endif
+#ifndef SPC5_TILE_STREAM
+#define SPC5_TILE_STREAM 1
 #endif
 #ifndef SPC5_XCD_REMAP
 #define SPC5_XCD_REMAP 1
@@ -35,7 +38,8 @@ static_assert(kMaxRows <= 256, "tile-local rows are stored as bytes");
 static_assert(kExpandCap % 64 == 0 && kExpandCap >= 128, "expansion cap must hold a full block");
 
 // Output stage of a product: y = alpha * (A x) + beta * y (y is not read
-// when beta == 0); with dv != null, sum(dv[i] * y_new[i]) is added to *dot.
+// when beta == 0); with dv != null, tile t writes sum(dv[i] * y_new[i]) over
+// its rows to dot[t] (one partial per tile, summed by the caller).
 struct Epilogue {
     double alpha = 1.0;
     double beta = 1.0;

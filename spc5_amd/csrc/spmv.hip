@@ -47,13 +47,24 @@ template <class V, class X, class ACC, int R, int VS, int MB>
 __global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_tiles(
     const int* __restrict__ bptr, const unsigned char* __restrict__ hdr, const V* __restrict__ val,
     const X* __restrict__ x, X* __restrict__ y, const int* __restrict__ tile_br,
-    const int* __restrict__ tile_val, int64_t ntiles, int64_t nrows, Epilogue ep) {
+    const int* __restrict__ tile_val, int64_t ntiles, int64_t nrows, int64_t ncols, Epilogue ep) {
+#if SPC5_TILE_STREAM
+    __shared__ StreamSmem<X, ACC, R, VS> sm[kWavesPerGroup];
+#else
     __shared__ TileSmem<ACC, R> sm[kWavesPerGroup];
+#endif
     const int w = threadIdx.x >> 6;
     const int64_t t = xcd_tile(w);
     if (t >= ntiles) return;  // whole wave leaves; no workgroup barrier is used
-    spmv_tile<V, X, ACC, R, VS, MB>(sm[w], bptr, hdr, val, x, y, tile_br[t], tile_br[t + 1], tile_val[t], nrows,
-                                    ep);
+    Epilogue e = ep;
+    if (e.dot) e.dot += t;  // one partial per tile
+#if SPC5_TILE_STREAM
+    spmv_tile_stream<V, X, ACC, R, VS, MB>(sm[w], bptr, hdr, val, x, y, tile_br[t], tile_br[t + 1], tile_val[t],
+                                           nrows, ncols, e);
+#else
+    (void)ncols;
+    spmv_tile<V, X, ACC, R, VS, MB>(sm[w], bptr, hdr, val, x, y, tile_br[t], tile_br[t + 1], tile_val[t], nrows, e);
+#endif
 }
 
 // ---- plan kernels --------------------------------------------------------
@@ -147,19 +158,19 @@ int launch_spmv_vs(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s,
     switch (m.r) {
     case 1:
         hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 1, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
-                           tb, tv, m.plan.ntiles, m.nrows, ep);
+                           tb, tv, m.plan.ntiles, m.nrows, m.ncols, ep);
         break;
     case 2:
         hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 2, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
-                           tb, tv, m.plan.ntiles, m.nrows, ep);
+                           tb, tv, m.plan.ntiles, m.nrows, m.ncols, ep);
         break;
     case 4:
         hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 4, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
-                           tb, tv, m.plan.ntiles, m.nrows, ep);
+                           tb, tv, m.plan.ntiles, m.nrows, m.ncols, ep);
         break;
     case 8:
         hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 8, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
-                           tb, tv, m.plan.ntiles, m.nrows, ep);
+                           tb, tv, m.plan.ntiles, m.nrows, m.ncols, ep);
         break;
     default:
         set_error("invalid r");

@@ -22,7 +22,7 @@
 namespace spc5 {
 
 #ifndef SPC5_PIPELINE
-#define SPC5_PIPELINE 1
+#define SPC5_PIPELINE 0
 #endif
 
 template <class ACC, int R>
@@ -137,6 +137,160 @@ __device__ __forceinline__ void accumulate(ACC* acc, int total, const ACC* a, co
     }
 }
 
+// ---------------------------------------------------------------------------
+// Block-stream tile (the default).  Per batch of up to 64 blocks:
+//   * values of the batch: coalesced loads from the stream into LDS;
+//   * each taken lane loads its block's x window x[start .. start+VS) (the
+//     reference's unaligned VS-wide x load) into its own LDS row;
+//   * each lane then walks its block's set mask bits in stream order (rows
+//     in order, columns ascending): sum += value * x_window[bit % VS] from
+//     LDS, and adds the sum to the tile's row accumulator when the row
+//     changes.
+// No per-value decode, gather or cross-lane reduction: the work per value is
+// two LDS reads and an FMA.
+// ---------------------------------------------------------------------------
+template <class X, class ACC, int R, int VS>
+struct StreamSmem {
+    static constexpr int kXStride = VS + 1;  // padded per-lane x window (bank spread)
+    ACC acc[kMaxRows];
+    int rs[kMaxRows / R + 1];
+    ACC val[kExpandCap];
+    X xw[64 * kXStride];
+};
+
+template <class V, class X, class ACC, int R, int VS, int MB>
+__device__ __forceinline__ void spmv_tile_stream(StreamSmem<X, ACC, R, VS>& sm, const int* __restrict__ bptr,
+                                                 const unsigned char* __restrict__ hdr, const V* __restrict__ val,
+                                                 const X* __restrict__ x, X* __restrict__ y, int br0, int br1,
+                                                 int vstart, int64_t nrows, int64_t ncols, const Epilogue& ep) {
+    constexpr int K = kExpandCap / 64;
+    constexpr int NW = MaskWords<R * MB>::kWords;
+    constexpr int XS = StreamSmem<X, ACC, R, VS>::kXStride;
+    const int lane = lane_id();
+
+    const int nbr = br1 - br0;
+    const int b0 = bptr[br0];
+    const int b1 = bptr[br1];
+    int v = vstart;
+    const int64_t row0 = (int64_t)br0 * R;
+    const int64_t rows_left = nrows - row0;
+    const int nrow_t = (int)((int64_t)nbr * R < rows_left ? (int64_t)nbr * R : rows_left);
+
+    for (int i = lane; i <= nbr; i += 64) sm.rs[i] = bptr[br0 + i] - b0;
+    for (int i = lane; i < nbr * R; i += 64) sm.acc[i] = (ACC)0;
+    int top = 1;
+    while (top < nbr) top <<= 1;
+
+    int b = b0;
+    int cs;
+    unsigned long long mw[NW];
+    load_header<R, MB>(hdr, b + lane, b1, cs, mw);
+    wave_sync();
+    X* xw = &sm.xw[lane * XS];
+
+    while (b < b1) {
+        const BatchShape bs = batch_shape<NW>(mw, b + lane < b1);
+        // values of the batch -> LDS (coalesced)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (64 * k >= bs.total) break;  // wave-uniform
+            const int q = lane + 64 * k;
+            if (q < bs.total) sm.val[q] = ValueLoad<V, ACC>::get(val + v + q);
+        }
+        const bool taken = lane < bs.ntake;
+        const int jcur = b + lane;
+        const int cs_cur = cs;
+        unsigned long long m_cur[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) m_cur[k] = mw[k];
+        // the block's x window -> LDS
+        if (taken) {
+            if ((int64_t)cs_cur + VS <= ncols) {
+                X w[VS];
+                __builtin_memcpy(w, x + cs_cur, sizeof(w));
+#pragma unroll
+                for (int j = 0; j < VS; ++j) xw[j] = w[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < VS; ++j) {
+                    const int64_t c = (int64_t)cs_cur + j < ncols ? (int64_t)cs_cur + j : ncols - 1;
+                    xw[j] = x[c];
+                }
+            }
+        }
+        const int bn = b + bs.ntake;
+        load_header<R, MB>(hdr, bn + lane, b1, cs, mw);
+        const int rbase = taken ? find_blockrow(sm.rs, nbr, top, jcur - b0) * R : 0;
+        wave_sync();
+        if (taken) {
+            int p = bs.incl - bs.cnt;
+            ACC sum = (ACC)0;
+            if constexpr (R == 1 && R * MB <= 4) {
+                unsigned m = (unsigned)m_cur[0];
+                while (m) {
+                    const int bit = __builtin_ctz(m);
+                    sum = __builtin_fma(sm.val[p], (ACC)xw[bit], sum);
+                    ++p;
+                    m &= m - 1u;
+                }
+                if (bs.cnt) atomicAdd(&sm.acc[rbase], sum);
+            } else if constexpr (R * MB <= 4) {
+                unsigned m = (unsigned)m_cur[0];
+                int cur = -1;
+                while (m) {
+                    const int bit = __builtin_ctz(m);
+                    const int s = bit / VS;
+                    if (s != cur) {
+                        if (cur >= 0) atomicAdd(&sm.acc[rbase + cur], sum);
+                        cur = s;
+                        sum = (ACC)0;
+                    }
+                    sum = __builtin_fma(sm.val[p], (ACC)xw[bit & (VS - 1)], sum);
+                    ++p;
+                    m &= m - 1u;
+                }
+                if (cur >= 0) atomicAdd(&sm.acc[rbase + cur], sum);
+            } else {
+                int cur = -1;
+#pragma unroll
+                for (int wd = 0; wd < NW; ++wd) {
+                    unsigned long long m = m_cur[wd];
+                    while (m) {
+                        const int bit = __builtin_ctzll(m) + wd * 64;
+                        const int s = bit / VS;
+                        if (s != cur) {
+                            if (cur >= 0) atomicAdd(&sm.acc[rbase + cur], sum);
+                            cur = s;
+                            sum = (ACC)0;
+                        }
+                        sum = __builtin_fma(sm.val[p], (ACC)xw[bit & (VS - 1)], sum);
+                        ++p;
+                        m &= m - 1ull;
+                    }
+                }
+                if (cur >= 0) atomicAdd(&sm.acc[rbase + cur], sum);
+            }
+        }
+        v += bs.total;
+        b = bn;
+        wave_sync();
+    }
+    wave_sync();
+    const ACC alpha = (ACC)ep.alpha, beta = (ACC)ep.beta;
+    const X* __restrict__ dv = reinterpret_cast<const X*>(ep.dv);
+    double part = 0.0;
+    for (int i = lane; i < nrow_t; i += 64) {
+        const ACC ax = sm.acc[i];
+        const X yn = ep.beta == 0.0 ? (X)(alpha * ax) : (X)(alpha * ax + beta * (ACC)y[row0 + i]);
+        y[row0 + i] = yn;
+        if (dv) part += (double)dv[row0 + i] * (double)yn;
+    }
+    if (ep.dot) {
+        part = wave_sum_f64(part);
+        if (lane == 0) ep.dot[0] = part;
+    }
+}
+
 template <class V, class X, class ACC, int R, int VS, int MB>
 __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __restrict__ bptr,
                                           const unsigned char* __restrict__ hdr, const V* __restrict__ val,
@@ -243,7 +397,7 @@ __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __res
     }
     if (ep.dot) {
         part = wave_sum_f64(part);
-        if (lane == 0) atomicAdd(ep.dot, part);
+        if (lane == 0) ep.dot[0] = part;
     }
 }
 
