@@ -53,6 +53,7 @@ struct spc5_hip_batch {
 namespace spc5 {
 namespace {
 
+#if SPC5_TILE_STREAM
 template <class X, class ACC, int VS>
 union BatchSmem {
     StreamSmem<X, ACC, 1, VS> r1;
@@ -60,6 +61,19 @@ union BatchSmem {
     StreamSmem<X, ACC, 4, VS> r4;
     StreamSmem<X, ACC, 8, VS> r8;
 };
+#define SPC5_BATCH_TILE(RR, SMEM)                                                                             \
+    spmv_tile_stream<V, X, ACC, RR, VS, MB>(SMEM, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, m.ncols, ep)
+#else
+template <class X, class ACC, int VS>
+union BatchSmem {
+    TileSmem<ACC, 1> r1;
+    TileSmem<ACC, 2> r2;
+    TileSmem<ACC, 4> r4;
+    TileSmem<ACC, 8> r8;
+};
+#define SPC5_BATCH_TILE(RR, SMEM) \
+    spmv_tile<V, X, ACC, RR, VS, MB>(SMEM, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, ep)
+#endif
 
 template <class V, class X, class ACC, int VS, int MB>
 __global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_batch(const BatchEntry* __restrict__ mats,
@@ -79,20 +93,16 @@ __global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_batch(const BatchE
         X* y = reinterpret_cast<X*>(m.y);
         switch (m.r) {
         case 1:
-            spmv_tile_stream<V, X, ACC, 1, VS, MB>(sm[w].r1, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, m.ncols,
-                                                   ep);
+            SPC5_BATCH_TILE(1, sm[w].r1);
             break;
         case 2:
-            spmv_tile_stream<V, X, ACC, 2, VS, MB>(sm[w].r2, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, m.ncols,
-                                                   ep);
+            SPC5_BATCH_TILE(2, sm[w].r2);
             break;
         case 4:
-            spmv_tile_stream<V, X, ACC, 4, VS, MB>(sm[w].r4, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, m.ncols,
-                                                   ep);
+            SPC5_BATCH_TILE(4, sm[w].r4);
             break;
         default:
-            spmv_tile_stream<V, X, ACC, 8, VS, MB>(sm[w].r8, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, m.ncols,
-                                                   ep);
+            SPC5_BATCH_TILE(8, sm[w].r8);
             break;
         }
         wave_sync();

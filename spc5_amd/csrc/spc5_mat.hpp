@@ -26,7 +26,7 @@ namespace spc5 {
 #define SPC5_SEGREDUCE 1
 #endif
 #ifndef SPC5_TILE_STREAM
-#define SPC5_TILE_STREAM 1
+#define SPC5_TILE_STREAM 0
 #endif
 #ifndef SPC5_XCD_REMAP
 #define SPC5_XCD_REMAP 1

@@ -385,7 +385,7 @@ __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __res
 #endif
     wave_sync();
     // y = alpha * (A x) + beta * y; y is not read when beta == 0.  With a dot
-    // vector, the tile also adds sum(dv[i] * y_new[i]) to *ep.dot.
+    // vector, the tile writes sum(dv[i] * y_new[i]) over its rows to *ep.dot.
     const ACC alpha = (ACC)ep.alpha, beta = (ACC)ep.beta;
     const X* __restrict__ dv = reinterpret_cast<const X*>(ep.dv);
     double part = 0.0;
