@@ -135,14 +135,15 @@ def cpu_baseline(rp, col, val, r, ncols, nthreads, nloops=16, max_rows=None):
 
 def solve_benchmark(n=160, r=4, rtol=1e-10):
     """Precision form on the 27-point stencil n^3 (diag 26.001, off -1, SPD):
-    wall time to relative residual rtol of double CG and of mixed-precision
-    IR-CG with float and bfloat16 values, plus errors against the double
+    wall time to relative residual rtol of double CG and of the two mixed-
+    precision solvers (restarted IR-CG and reliable-update CG) with float and
+    bfloat16 values, plus errors against the double
     solution and of one reduced-precision product against the double one."""
     import numpy as np
     import torch
     from spc5_amd import synth
     from spc5_amd.device import DeviceMatrix
-    from spc5_amd.solver import cg, ir_cg, lowp
+    from spc5_amd.solver import cg, ir_cg, lowp, mixed_cg
 
     rp, col, val = synth.stencil27(n)
     N = len(rp) - 1
@@ -176,14 +177,31 @@ def solve_benchmark(n=160, r=4, rtol=1e-10):
         y64 = torch.zeros(N, dtype=torch.float64, device="cuda")
         A64.spmv(bf.to(torch.float64), y64)
         spmv_err = float((ylo.double() - y64).abs().max() / (54.0 * float(bf.abs().max())))
+        ms_lo, _ = time_spmv(Alo, bf, ylo, 20, 3)
+        ms_64, _ = time_spmv(A64, bf.to(torch.float64), y64, 20, 3)
         out[f"ir_cg_{kind}"] = {
             "seconds": round(t, 4), "speedup_vs_fp64": round(t64 / t, 3), "inner_iterations": st["iterations"],
             "outer": st["outer"], "relres": st["relres"],
             "x_rel_diff_vs_fp64_cg": float(torch.linalg.vector_norm(x - x64)) / nx64,
             "spmv_normwise_err_vs_fp64": spmv_err,
-            "value_bytes": Alo.values.element_size(), "bytes_per_spmv": Alo.algorithmic_bytes()}
+            "value_bytes": Alo.values.element_size(), "bytes_per_spmv": Alo.algorithmic_bytes(),
+            "spmv_ms": round(ms_lo, 4), "fp64_spmv_ms": round(ms_64, 4)}
+        log(f"[bench] {kind} SpMV {ms_lo:.4f} ms vs fp64 {ms_64:.4f} ms")
         log(f"[bench] IR-CG {kind}: {t * 1e3:.1f} ms ({t64 / t:.2f}x), {st['outer']} outer / "
             f"{st['iterations']} inner, relres {st['relres']:.2e}")
+        # reliable-update CG on the same pair of matrices
+        mixed_cg(A64, Alo, b, rtol=1e-2, maxit=16)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, st = mixed_cg(A64, Alo, b, rtol=rtol)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        out[f"mixed_cg_{kind}"] = {
+            "seconds": round(t, 4), "speedup_vs_fp64": round(t64 / t, 3), "iterations": st["iterations"],
+            "reliable_updates": st["outer"], "relres": st["relres"],
+            "x_rel_diff_vs_fp64_cg": float(torch.linalg.vector_norm(x - x64)) / nx64}
+        log(f"[bench] mixed CG {kind}: {t * 1e3:.1f} ms ({t64 / t:.2f}x), {st['iterations']} it / "
+            f"{st['outer']} reliable updates, relres {st['relres']:.2e}")
         del Alo
     out["fp64_bytes_per_spmv"] = A64.algorithmic_bytes()
     del A64
@@ -308,6 +326,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-max-rows", type=int, default=0)
     ap.add_argument("--no-copy", action="store_true")
+    ap.add_argument("--no-f32", action="store_true", help="skip the FP32 variant of the SpMV matrix")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--solve-n", type=int, default=160)
     ap.add_argument("--no-spmm", action="store_true")
@@ -363,6 +382,26 @@ def main():
         del m
         torch.cuda.empty_cache()
 
+    # FP32 variant of the same matrix: float values, reference float format beta(r,16)
+    formats_f32 = {}
+    if not args.no_f32:
+        d_val32 = d_val.to(torch.float32)
+        x32 = x.to(torch.float32)
+        y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
+        for r in [int(s) for s in args.formats.split(",")]:
+            m = DeviceMatrix.from_csr(d_rp, d_col, d_val32, r)
+            ms, _ = time_spmv(m, x32, y32, args.steps, args.warmup)
+            B = m.algorithmic_bytes()
+            gbs = B / (ms * 1e-3) / 1e9
+            formats_f32[m.format_name] = {"r": r, "vs": m.vs, "nblocks": m.nblocks, "bytes_per_spmv": B,
+                                          "ms": round(ms, 5), "gflops": round(2.0 * nnz / (ms * 1e-3) / 1e9, 2),
+                                          "gbs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
+            log(f"[bench] FP32 beta({r},{m.vs}): {ms:.4f} ms  {gbs:.0f} GB/s  "
+                f"({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)")
+            del m
+        del d_val32, x32, y32
+        torch.cuda.empty_cache()
+
     bf = formats[best]
     copy_gbs = None if args.no_copy else round(copy_bandwidth(), 1)
     out = {
@@ -384,6 +423,7 @@ def main():
                      "frac": bf["frac_of_peak"],
                      "traffic": traffic_from_profiles(args.workload, best)},
         "formats": formats,
+        "formats_f32": formats_f32,
         "copy_gbs_measured": copy_gbs,
     }
     del d_rp, d_col, d_val, x, y

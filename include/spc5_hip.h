@@ -189,6 +189,17 @@ int spc5_hip_cg(spc5_hip_mat_t A, const void* d_b, void* d_x, double rtol, int m
 int spc5_hip_ir_cg(spc5_hip_mat_t A64, spc5_hip_mat_t Alo, const double* d_b, double* d_x, double rtol,
                    double inner_rtol, int max_outer, int max_inner, spc5_hip_solve_stats* st, void* stream);
 
+/* Mixed CG with reliable updates: CG runs on float vectors with A_lo (float
+ * or bfloat16 values) and accumulates a float correction d.  Every 8
+ * iterations, when the iterated residual has dropped by `delta` (0 < delta
+ * < 1) since the last update, d is folded into x (double), r = b - A64 x is
+ * recomputed with the double SpMV and replaces the iterated residual, and
+ * CG goes on with its search direction kept.  Stops at
+ * ||b - A64 x|| <= rtol ||b|| or after maxit float iterations;
+ * st->iterations counts float iterations, st->outer reliable updates. */
+int spc5_hip_mixed_cg(spc5_hip_mat_t A64, spc5_hip_mat_t Alo, const double* d_b, double* d_x, double rtol,
+                      double delta, int maxit, spc5_hip_solve_stats* st, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Batching form: many independent matrices in one persistent launch      */
 /* (no counterpart in the reference).                                     */

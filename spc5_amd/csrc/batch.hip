@@ -53,17 +53,6 @@ struct spc5_hip_batch {
 namespace spc5 {
 namespace {
 
-#if SPC5_TILE_STREAM
-template <class X, class ACC, int VS>
-union BatchSmem {
-    StreamSmem<X, ACC, 1, VS> r1;
-    StreamSmem<X, ACC, 2, VS> r2;
-    StreamSmem<X, ACC, 4, VS> r4;
-    StreamSmem<X, ACC, 8, VS> r8;
-};
-#define SPC5_BATCH_TILE(RR, SMEM)                                                                             \
-    spmv_tile_stream<V, X, ACC, RR, VS, MB>(SMEM, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, m.ncols, ep)
-#else
 template <class X, class ACC, int VS>
 union BatchSmem {
     TileSmem<ACC, 1> r1;
@@ -72,8 +61,7 @@ union BatchSmem {
     TileSmem<ACC, 8> r8;
 };
 #define SPC5_BATCH_TILE(RR, SMEM) \
-    spmv_tile<V, X, ACC, RR, VS, MB>(SMEM, m.bptr, m.hdr, val, x, y, br0, br1, v0, m.nrows, ep)
-#endif
+    spmv_tile<V, X, ACC, RR, VS, MB>(SMEM, m.bptr, m.hdr, val, x, y, br0, br1, v0, v1, m.nrows, ep)
 
 template <class V, class X, class ACC, int VS, int MB>
 __global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_batch(const BatchEntry* __restrict__ mats,
@@ -87,7 +75,7 @@ __global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_batch(const BatchE
     for (int64_t t = (int64_t)blockIdx.x * kWavesPerGroup + w; t < ntiles; t += stride) {
         const BatchEntry& m = mats[tile_mat[t]];
         const int lt = tile_loc[t];
-        const int br0 = m.tile_br[lt], br1 = m.tile_br[lt + 1], v0 = m.tile_val[lt];
+        const int br0 = m.tile_br[lt], br1 = m.tile_br[lt + 1], v0 = m.tile_val[lt], v1 = m.tile_val[lt + 1];
         const V* val = reinterpret_cast<const V*>(m.val);
         const X* x = reinterpret_cast<const X*>(m.x);
         X* y = reinterpret_cast<X*>(m.y);

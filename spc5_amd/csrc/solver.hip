@@ -17,6 +17,15 @@
 //   loose tolerance (A_lo: the same block structure with float or bfloat16
 //   values, float accumulation);  x64 += ||r64|| d;  r64 = b - A64 x64
 //   (double SpMV) -- until ||r64|| <= rtol ||b||.
+//
+// Mixed CG with reliable updates: one CG iteration runs on float vectors
+// with A_lo and accumulates a float correction d; whenever the iterated
+// residual has dropped by a factor delta since the last update (checked
+// every `check` iterations), d is folded into the double solution
+// (x64 += d, d = 0), the residual is recomputed as b - A64 x64 with the
+// double SpMV and replaces the iterated one, and the iteration goes on with
+// its search direction p kept (beta uses the new residual).  Unlike the
+// restarted refinement above, the Krylov space is never thrown away.
 #include "spc5_mat.hpp"
 
 #include <algorithm>
@@ -126,6 +135,39 @@ __global__ __launch_bounds__(kVecThreads) void k_cg_xp(X* __restrict__ x, X* __r
         x[i] += alpha * pi;
         p[i] = r[i] + beta * pi;
     }
+}
+
+// K3 split in two around a reliable update: x += alpha p ...
+template <class X>
+__global__ __launch_bounds__(kVecThreads) void k_cg_x(X* __restrict__ x, const X* __restrict__ p, int64_t n,
+                                                      const CgScalars* __restrict__ sc) {
+    const X alpha = (X)(sc->rr / sc->pq);
+    for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads)
+        x[i] += alpha * p[i];
+}
+
+// ... and p = r + beta p
+template <class X>
+__global__ __launch_bounds__(kVecThreads) void k_cg_p(X* __restrict__ p, const X* __restrict__ r, int64_t n,
+                                                      const CgScalars* __restrict__ sc) {
+    const X beta = (X)(sc->rr_new / sc->rr);
+    for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads)
+        p[i] = r[i] + beta * p[i];
+}
+
+// x64 += d; d = 0
+__global__ __launch_bounds__(kVecThreads) void k_fold_f32(double* __restrict__ x, float* __restrict__ d, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads) {
+        x[i] += (double)d[i];
+        d[i] = 0.0f;
+    }
+}
+
+// dst = (float) src
+__global__ __launch_bounds__(kVecThreads) void k_to_f32(float* __restrict__ dst, const double* __restrict__ src,
+                                                        int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads)
+        dst[i] = (float)src[i];
 }
 
 __global__ void k_cg_rotate(CgScalars* sc) { sc->rr = sc->rr_new; }
@@ -285,6 +327,114 @@ int spc5_hip_cg(spc5_hip_mat_t A, const void* d_b, void* d_x, double rtol, int m
     if (A->value_type == SPC5_HIP_F64)
         return cg_impl<double>(*A, (const double*)d_b, (double*)d_x, rtol, maxit, 8, st, s);
     return cg_impl<float>(*A, (const float*)d_b, (float*)d_x, rtol, maxit, 8, st, s);
+}
+
+int spc5_hip_mixed_cg(spc5_hip_mat_t A64, spc5_hip_mat_t Alo, const double* d_b, double* d_x, double rtol,
+                      double delta, int maxit, spc5_hip_solve_stats* st, void* stream) {
+    SPC5_CHECK_ARG(A64 != nullptr && Alo != nullptr && d_b != nullptr && d_x != nullptr, "null argument");
+    SPC5_CHECK_ARG(A64->value_type == SPC5_HIP_F64, "A64 must hold double values");
+    SPC5_CHECK_ARG(Alo->value_type != SPC5_HIP_F64, "Alo must hold float or bfloat16 values");
+    SPC5_CHECK_ARG(A64->nrows == Alo->nrows && A64->ncols == Alo->ncols && A64->nrows == A64->ncols,
+                   "A64 and Alo must be the same square matrix");
+    SPC5_CHECK_ARG(rtol > 0.0 && delta > 0.0 && delta < 1.0 && maxit >= 0, "tolerances out of range");
+    hipStream_t s = as_stream(stream);
+    const int64_t n = A64->nrows;
+    const int check = 8;
+    const int64_t npart = std::max<int64_t>(kMaxVecGroups, Alo->plan.ntiles);
+    DeviceBuffer vr64, vr, vp, vq, vd, vsc, vtmp, vpart;
+    SPC5_HIP_TRY(vr64.alloc(sizeof(double) * (size_t)n));
+    SPC5_HIP_TRY(vr.alloc(sizeof(float) * (size_t)n));
+    SPC5_HIP_TRY(vp.alloc(sizeof(float) * (size_t)n));
+    SPC5_HIP_TRY(vq.alloc(sizeof(float) * (size_t)n));
+    SPC5_HIP_TRY(vd.alloc(sizeof(float) * (size_t)n));
+    SPC5_HIP_TRY(vsc.alloc(sizeof(CgScalars)));
+    SPC5_HIP_TRY(vtmp.alloc(2 * sizeof(double)));
+    SPC5_HIP_TRY(vpart.alloc(sizeof(double) * (size_t)npart));
+    double* r64 = vr64.as<double>();
+    float* r = vr.as<float>();
+    float* p = vp.as<float>();
+    float* q = vq.as<float>();
+    float* d = vd.as<float>();
+    CgScalars* sc = vsc.as<CgScalars>();
+    double* part = vpart.as<double>();
+    double* d_bb = vtmp.as<double>();
+    double* d_rr = d_bb + 1;
+    const unsigned g = vec_grid(n);
+    int rc;
+    Epilogue eres;  // r64 = b - A64 x64 (r64 holds a copy of b)
+    eres.alpha = -1.0;
+    eres.beta = 1.0;
+    Epilogue ek1;  // q = A_lo p, partials of p.q
+    ek1.alpha = 1.0;
+    ek1.beta = 0.0;
+    ek1.dv = p;
+    ek1.dot = part;
+
+    if ((rc = dot<double>(d_b, d_b, n, part, d_bb, s))) return rc;
+    hipLaunchKernelGGL(k_copy<double>, dim3(g), dim3(kVecThreads), 0, s, r64, d_b, n);
+    if ((rc = launch_spmv(*A64, d_x, r64, s, eres))) return rc;
+    hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kVecThreads), 0, s, r, r64, n);
+    hipLaunchKernelGGL(k_copy<float>, dim3(g), dim3(kVecThreads), 0, s, p, r, n);
+    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kVecThreads), 0, s, d, n, 0.0f);
+    if ((rc = dot<double>(r64, r64, n, part, d_rr, s))) return rc;
+    hipLaunchKernelGGL(k_cg_init_scalars, dim3(1), dim3(1), 0, s, sc, d_rr);
+    SPC5_HIP_TRY(hipGetLastError());
+    double hbb = 0.0, hrr = 0.0;
+    if ((rc = read_scalar(d_bb, &hbb, s))) return rc;
+    if ((rc = read_scalar(d_rr, &hrr, s))) return rc;
+    if (hbb == 0.0) hbb = 1.0;
+    const double stop2 = rtol * rtol * hbb;
+    double rr_true = hrr;    // ||b - A64 x64||^2 at the last update
+    double rr_peak = hrr;    // largest iterated ||r||^2 seen since the last update
+    int it = 0, updates = 0;
+    while (rr_true > stop2 && it < maxit && std::isfinite(rr_true)) {
+        // `check` iterations; the last one stops before its x/p update
+        const int chunk = std::min(check, maxit - it);
+        for (int c = 0; c < chunk; ++c) {
+            if ((rc = launch_spmv(*Alo, p, q, s, ek1))) return rc;
+            hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kVecThreads), 0, s, part, Alo->plan.ntiles, &sc->pq);
+            hipLaunchKernelGGL(k_cg_r<float>, dim3(g), dim3(kVecThreads), 0, s, r, q, n, sc, part);
+            hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kVecThreads), 0, s, part, (int64_t)g, &sc->rr_new);
+            if (c + 1 < chunk) {
+                hipLaunchKernelGGL(k_cg_xp<float>, dim3(g), dim3(kVecThreads), 0, s, d, p, r, n, sc);
+                hipLaunchKernelGGL(k_cg_rotate, dim3(1), dim3(1), 0, s, sc);
+            }
+        }
+        SPC5_HIP_TRY(hipGetLastError());
+        it += chunk;
+        double rr_it = 0.0;
+        if ((rc = read_scalar(&sc->rr_new, &rr_it, s))) return rc;
+        if (!std::isfinite(rr_it)) {
+            rr_true = rr_it;
+            break;
+        }
+        rr_peak = std::max(rr_peak, rr_it);
+        hipLaunchKernelGGL(k_cg_x<float>, dim3(g), dim3(kVecThreads), 0, s, d, p, n, sc);
+        const bool update = rr_it <= delta * delta * rr_peak || rr_it <= stop2 || it >= maxit;
+        if (update) {
+            // reliable update: x64 += d, d = 0; r = b - A64 x64 (double)
+            hipLaunchKernelGGL(k_fold_f32, dim3(g), dim3(kVecThreads), 0, s, d_x, d, n);
+            hipLaunchKernelGGL(k_copy<double>, dim3(g), dim3(kVecThreads), 0, s, r64, d_b, n);
+            if ((rc = launch_spmv(*A64, d_x, r64, s, eres))) return rc;
+            hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kVecThreads), 0, s, r, r64, n);
+            if ((rc = dot<double>(r64, r64, n, part, &sc->rr_new, s))) return rc;
+            if ((rc = read_scalar(&sc->rr_new, &rr_true, s))) return rc;
+            ++updates;
+            rr_peak = rr_true;
+            if (rr_true <= stop2 || it >= maxit) break;
+        }
+        hipLaunchKernelGGL(k_cg_p<float>, dim3(g), dim3(kVecThreads), 0, s, p, r, n, sc);
+        hipLaunchKernelGGL(k_cg_rotate, dim3(1), dim3(1), 0, s, sc);
+        SPC5_HIP_TRY(hipGetLastError());
+    }
+    SPC5_HIP_TRY(hipStreamSynchronize(s));
+    if (st) {
+        st->iterations = it;
+        st->outer = updates;
+        st->relres = std::sqrt(rr_true / hbb);
+        st->recurrence_relres = 0.0;
+    }
+    return SPC5_HIP_OK;
 }
 
 int spc5_hip_ir_cg(spc5_hip_mat_t A64, spc5_hip_mat_t Alo, const double* d_b, double* d_x, double rtol,

@@ -61,6 +61,11 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Stops the compiler from moving memory accesses across this point (no
+// instruction is emitted).  Used to keep a group of independent loads
+// together ahead of their first use.
+__device__ __forceinline__ void compiler_fence() { asm volatile("" ::: "memory"); }
+
 // Inclusive prefix sum of an int over the 64 lanes: DPP shifts inside each
 // 16-lane row, then the three row totals are added with scalar readlanes.
 __device__ __forceinline__ int wave_incl_scan(int v) {

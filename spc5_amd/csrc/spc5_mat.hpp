@@ -25,16 +25,13 @@ namespace spc5 {
 #ifndef SPC5_SEGREDUCE
 #define SPC5_SEGREDUCE 1
 #endif
-#ifndef SPC5_TILE_STREAM
-#define SPC5_TILE_STREAM 0
-#endif
 #ifndef SPC5_XCD_REMAP
 #define SPC5_XCD_REMAP 1
 #endif
 constexpr int kMaxRows = SPC5_MAX_ROWS;              // scalar rows per tile (LDS accumulators), <= 256
 constexpr int kWavesPerGroup = SPC5_WAVES_PER_GROUP; // independent waves per workgroup
 constexpr int kExpandCap = SPC5_EXPAND_CAP;          // values expanded into LDS per batch
-static_assert(kMaxRows <= 256, "tile-local rows are stored as bytes");
+static_assert(kMaxRows <= 255, "tile-local rows are stored as bytes, kMaxRows marks none");
 static_assert(kExpandCap % 64 == 0 && kExpandCap >= 128, "expansion cap must hold a full block");
 
 // Output stage of a product: y = alpha * (A x) + beta * y (y is not read

@@ -3,15 +3,18 @@
 // dimension ldx) and Y (nrows x k, ldy).  The batching form; no counterpart
 // in the reference.
 //
-// One wave per (tile, column chunk).  The tile's blocks are decoded and
-// expanded exactly as in the SpMV tile (headers prefetched, values read
-// once, coalesced, from the value stream) and every value is then applied
-// to the kt columns of its X row: lanes are spread over (value, column)
-// pairs, KC lanes per value, so a chunk of kt <= 64 columns of one X row is
-// read contiguously.  With kt = k the matrix is fetched once for all k
-// columns; smaller kt re-reads the matrix k/kt times but keeps the x window
-// of each pass small enough to stay in L2 (the autotuner picks kt).
-// Per-row, per-column sums live in LDS (tiles are cut to fit them).
+// One wave per (tile, column chunk of kt <= 64 columns).  Per batch of up
+// to 64 blocks the tile's headers are decoded as in the SpMV tile (headers
+// prefetched, values read once, coalesced, from the value stream) and every
+// value becomes a 16-byte LDS record (value, column, tile-local row).  The
+// wave then splits into 64/KC groups of KC lanes (one lane per column of
+// the chunk); each group walks a contiguous run of the batch's records,
+// requests the X rows of 8 records at a time (one contiguous kt-wide row
+// segment per record) and keeps the running row sum in a register, adding
+// it to the tile's LDS row x column sums only when the row changes.  With
+// kt = k the matrix is fetched once for all k columns; smaller kt re-reads
+// the matrix k/kt times with a smaller X working set (the autotuner picks
+// kt).
 #include "spc5_mat.hpp"
 #include "spmv_tile.hpp"
 
@@ -24,14 +27,27 @@ namespace spc5 {
 namespace {
 
 constexpr int kSpmmCap = 256;        // values expanded per batch
-constexpr int kSpmmAccBytes = 16384;  // LDS for the row x column sums of a tile
+constexpr int kSpmmAccBytes = 8192;  // LDS for the row x column sums of a tile
+constexpr int kSpmmMaxKt = 64;       // columns per pass (one lane each)
+
+template <class ACC>
+struct alignas(16) SpmmRec {
+    ACC v;
+    int col;
+    int row;
+};
 
 template <class ACC>
 int spmm_max_rows(int kt) {
     int rows = kSpmmAccBytes / (kt * (int)sizeof(ACC));
-    if (rows > 256) rows = 256;
+    if (rows > 128) rows = 128;
     if (rows < 8) rows = 8;
     return rows;
+}
+
+template <class ACC>
+size_t spmm_smem_bytes(int maxrows, int kt, int r) {
+    return sizeof(SpmmRec<ACC>) * kSpmmCap + sizeof(ACC) * (size_t)maxrows * kt + sizeof(int) * (maxrows / r + 1);
 }
 
 template <class V, class X, class ACC, int R, int VS, int MB, int KC>
@@ -42,19 +58,23 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
                                                     int64_t ntiles, int64_t nrows, int maxrows, ACC alpha, ACC beta) {
     constexpr int K = kSpmmCap / 64;
     constexpr int NW = MaskWords<R * MB>::kWords;
-    constexpr int VPL = 64 / KC;  // values per step
+    constexpr int G = 64 / KC;  // record groups per wave
+    constexpr int U = 8;        // records whose X rows are requested together
+    using Rec = SpmmRec<ACC>;
     extern __shared__ __align__(16) unsigned char smem[];
-    ACC* acc = reinterpret_cast<ACC*>(smem);                          // [maxrows * kt]
-    ACC* sval = acc + (size_t)maxrows * kt;                           // [kSpmmCap]
-    int* scol = reinterpret_cast<int*>(sval + kSpmmCap);              // [kSpmmCap]
-    int* rs = scol + kSpmmCap;                                        // [maxrows / R + 1]
-    unsigned char* srow = reinterpret_cast<unsigned char*>(rs + maxrows / R + 1);  // [kSpmmCap]
+    Rec* rec = reinterpret_cast<Rec*>(smem);                            // [kSpmmCap]
+    ACC* acc = reinterpret_cast<ACC*>(rec + kSpmmCap);                  // [maxrows * kt]
+    int* rs = reinterpret_cast<int*>(acc + (size_t)maxrows * kt);       // [maxrows / R + 1]
 
     const int64_t t = blockIdx.x;
     if (t >= ntiles) return;
     const int c0 = blockIdx.y * kt;
     const int ktc = (k - c0) < kt ? (k - c0) : kt;
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
+    const int grp = lane / KC;
+    const int jl = lane % KC;
+    const bool colok = jl < ktc;
+    const X* __restrict__ xcol = x + c0 + (colok ? jl : 0);
     const int br0 = tile_br[t], br1 = tile_br[t + 1];
     const int nbr = br1 - br0;
     const int b0 = bptr[br0], b1 = bptr[br1];
@@ -69,60 +89,91 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
     while (top < nbr) top <<= 1;
 
     int b = b0;
-    int cs;
+    int cs = 0;
     unsigned long long mw[NW];
-    load_header<R, MB>(hdr, b + lane, b1, cs, mw);
+#pragma unroll
+    for (int w = 0; w < NW; ++w) mw[w] = 0ull;
+    if (b0 < b1) load_header_clamped<R, MB>(hdr, b + lane, b1 - 1, cs, mw);
     wave_sync();
-    const int sub = lane / KC;  // value slot of this lane inside a step
-    const int jl = lane % KC;   // column of this lane inside a pass
     while (b < b1) {
-        // shape of the batch (at most kSpmmCap values)
-        int cnt = 0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) cnt += __popcll(mw[w]);
-        const int incl = wave_incl_scan(cnt);
-        const int ntake = __popcll(ballot64(b + lane < b1 && incl <= kSpmmCap));
-        const int total = __builtin_amdgcn_readlane(incl, ntake - 1);
-#pragma unroll
-        for (int kk = 0; kk < K; ++kk) {
-            const int q = lane + 64 * kk;
-            if (q < total) sval[q] = ValueLoad<V, ACC>::get(val + v + q);
-        }
-        const int jcur = b + lane;
-        const int cs_cur = cs;
+        const bool valid = b + lane < b1;
         unsigned long long mw_cur[NW];
 #pragma unroll
-        for (int w = 0; w < NW; ++w) mw_cur[w] = mw[w];
+        for (int w = 0; w < NW; ++w) mw_cur[w] = valid ? mw[w] : 0ull;
+        int cs_cur = valid ? cs : 0;
+        asm volatile("" : "+v"(cs_cur));
+        int cnt = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) cnt += __popcll(mw_cur[w]);
+        const int incl = wave_incl_scan(cnt);
+        const int ntake = __popcll(ballot64(valid && incl <= kSpmmCap));
+        const int total = __builtin_amdgcn_readlane(incl, ntake - 1);
+        ACC a[K];
+        load_values<V, ACC, K>(val, v, total, lane, a);
+        const int jcur = b + lane;
         const int bn = b + ntake;
-        load_header<R, MB>(hdr, bn + lane, b1, cs, mw);
-        if (lane < ntake) expand<R, VS, NW>(scol, srow, rs, nbr, top, jcur - b0, cs_cur, mw_cur, incl - cnt);
-        wave_sync();
-        // apply every value of the batch to its X row chunk
-        for (int q0 = 0; q0 < total; q0 += VPL * 4) {
-            int qs[4];
+        load_header_clamped<R, MB>(hdr, bn + lane, b1 - 1, cs, mw);
+        if (lane < ntake) {
+            // records (column, tile-local row) of this block's set bits
+            const int rbase = find_blockrow(rs, nbr, top, jcur - b0) * R;
+            int p = incl - cnt;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) qs[u] = q0 + u * VPL + sub;
-            for (int j = jl; j < ktc; j += KC) {
-                ACC xx[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    xx[u] = qs[u] < total ? (ACC)x[(int64_t)scol[qs[u]] * ldx + c0 + j] : (ACC)0;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (qs[u] < total) {
-                        ACC* a = &acc[(int)srow[qs[u]] * kt + j];
-                        if (VPL == 1)
-                            *a += sval[qs[u]] * xx[u];
-                        else
-                            atomicAdd(a, sval[qs[u]] * xx[u]);
-                    }
+            for (int w = 0; w < NW; ++w) {
+                unsigned long long m = mw_cur[w];
+                while (m) {
+                    const int bit = __builtin_ctzll(m) + w * 64;
+                    rec[p].col = cs_cur + (bit & (VS - 1));
+                    rec[p].row = rbase + bit / VS;
+                    ++p;
+                    m &= m - 1ull;
                 }
             }
         }
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            const int q = lane + 64 * kk;
+            if (q < total) rec[q].v = a[kk];
+        }
+        wave_sync();
+        // group grp walks records [qa, qb)
+        const int per = (total + G - 1) / G;
+        int q = grp * per;
+        const int qb = q + per < total ? q + per : total;
+        ACC run = (ACC)0;
+        int prow = -1;
+        for (; q + U <= qb; q += U) {
+            Rec rr[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) rr[u] = rec[q + u];
+            ACC xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = (ACC)xcol[(int64_t)rr[u].col * ldx];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (rr[u].row != prow) {
+                    if (prow >= 0 && colok) atomicAdd(&acc[prow * kt + jl], run);
+                    run = (ACC)0;
+                    prow = rr[u].row;
+                }
+                run += rr[u].v * xv[u];
+            }
+        }
+        for (; q < qb; ++q) {
+            const Rec r1 = rec[q];
+            const ACC xv = (ACC)xcol[(int64_t)r1.col * ldx];
+            if (r1.row != prow) {
+                if (prow >= 0 && colok) atomicAdd(&acc[prow * kt + jl], run);
+                run = (ACC)0;
+                prow = r1.row;
+            }
+            run += r1.v * xv;
+        }
+        if (prow >= 0 && colok) atomicAdd(&acc[prow * kt + jl], run);
         v += total;
         b = bn;
         wave_sync();
     }
+    wave_sync();
     for (int i = lane; i < nrow_t * ktc; i += 64) {
         const int rr = i / ktc, j = i % ktc;
         X* yp = &y[(row0 + rr) * ldy + c0 + j];
@@ -144,8 +195,7 @@ template <class V, class X, class ACC, int VS, int MB, int KC>
 int launch_spmm_kc(const spc5_hip_mat& m, const Plan& p, int maxrows, const void* x, int64_t ldx, void* y,
                    int64_t ldy, int k, int kt, double alpha, double beta, hipStream_t s) {
     const dim3 grid((unsigned)p.ntiles, (unsigned)((k + kt - 1) / kt));
-    const size_t shm = sizeof(ACC) * ((size_t)maxrows * kt + kSpmmCap) + sizeof(int) * (kSpmmCap + maxrows + 1) +
-                       kSpmmCap + 16;
+    const size_t shm = spmm_smem_bytes<ACC>(maxrows, kt, m.r);
     const int* tb = p.tile_br->as<int>();
     const int* tv = p.tile_val->as<int>();
     const V* val = reinterpret_cast<const V*>(m.values);
@@ -187,7 +237,8 @@ int launch_spmm_t(const spc5_hip_mat& m, const void* x, int64_t ldx, void* y, in
         p = &it->second;
     }
     if (p->ntiles == 0) return SPC5_HIP_OK;
-    const int kc = kt >= 64 ? 64 : (kt >= 32 ? 32 : (kt >= 16 ? 16 : (kt >= 8 ? 8 : (kt >= 4 ? 4 : (kt >= 2 ? 2 : 1)))));
+    int kc = 1;  // lanes per record: the smallest power of two holding the chunk's kt columns
+    while (kc < kt) kc <<= 1;
     switch (kc) {
     case 64: return launch_spmm_kc<V, X, ACC, VS, MB, 64>(m, *p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s);
     case 32: return launch_spmm_kc<V, X, ACC, VS, MB, 32>(m, *p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s);
@@ -217,6 +268,7 @@ int spc5_hip_spmm(spc5_hip_mat_t h, int k, int kt, double alpha, const void* d_x
     SPC5_CHECK_ARG(k >= 1 && ldx >= k && ldy >= k, "need k >= 1, ldx >= k, ldy >= k");
     SPC5_CHECK_ARG(d_x != nullptr && d_y != nullptr, "x or y is null");
     if (kt <= 0 || kt > k) kt = k;
+    if (kt > kSpmmMaxKt) kt = kSpmmMaxKt;
     hipStream_t s = as_stream(stream);
     if (h->value_type == SPC5_HIP_F64)
         return launch_spmm_t<double, double, double, 8, 1>(*h, d_x, ldx, d_y, ldy, k, kt, alpha, beta, s);

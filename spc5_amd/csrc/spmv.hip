@@ -48,23 +48,15 @@ __global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_tiles(
     const int* __restrict__ bptr, const unsigned char* __restrict__ hdr, const V* __restrict__ val,
     const X* __restrict__ x, X* __restrict__ y, const int* __restrict__ tile_br,
     const int* __restrict__ tile_val, int64_t ntiles, int64_t nrows, int64_t ncols, Epilogue ep) {
-#if SPC5_TILE_STREAM
-    __shared__ StreamSmem<X, ACC, R, VS> sm[kWavesPerGroup];
-#else
     __shared__ TileSmem<ACC, R> sm[kWavesPerGroup];
-#endif
     const int w = threadIdx.x >> 6;
     const int64_t t = xcd_tile(w);
     if (t >= ntiles) return;  // whole wave leaves; no workgroup barrier is used
     Epilogue e = ep;
     if (e.dot) e.dot += t;  // one partial per tile
-#if SPC5_TILE_STREAM
-    spmv_tile_stream<V, X, ACC, R, VS, MB>(sm[w], bptr, hdr, val, x, y, tile_br[t], tile_br[t + 1], tile_val[t],
-                                           nrows, ncols, e);
-#else
     (void)ncols;
-    spmv_tile<V, X, ACC, R, VS, MB>(sm[w], bptr, hdr, val, x, y, tile_br[t], tile_br[t + 1], tile_val[t], nrows, e);
-#endif
+    spmv_tile<V, X, ACC, R, VS, MB>(sm[w], bptr, hdr, val, x, y, tile_br[t], tile_br[t + 1], tile_val[t],
+                                    tile_val[t + 1], nrows, e);
 }
 
 // ---- plan kernels --------------------------------------------------------

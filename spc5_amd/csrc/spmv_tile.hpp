@@ -1,36 +1,66 @@
-// spmv_tile.hpp -- one wave computes y += A x over one tile (a run of whole
-// block-rows) of a beta(r,VS) matrix in the reference layout.
+// spmv_tile.hpp -- one wave computes y = alpha A x + beta y over one tile (a
+// run of whole block-rows) of a beta(r,VS) matrix in the reference layout.
 //
 // Per batch of up to 64 blocks (one block per lane):
-//   1. the lane's block header (int32 first column + r masks) was loaded in
-//      the previous iteration (prefetch); popcounts and a DPP wave scan give
-//      every block's value offset inside the batch; blocks are taken while
-//      the batch holds at most kExpandCap values;
+//   1. the lane's block header (int32 first column + r masks) was loaded
+//      ahead; popcounts and a DPP wave scan give every block's value offset
+//      inside the batch; blocks are taken while the batch holds at most
+//      kExpandCap values;
 //   2. the batch's values are loaded at once, kExpandCap/64 coalesced loads
 //      per lane, straight from the value stream (no zero padding);
-//   3. the next batch's headers are requested;
+//   3. later headers are requested (SPC5_PIPE: steps 1-3 run one batch
+//      ahead, so a batch's values are in flight while the previous batch
+//      goes through steps 4-6);
 //   4. each taken lane finds its block-row (binary search over the tile's
 //      block-row pointers in LDS) and writes, for every set mask bit, the
 //      column and the tile-local row into LDS;
-//   5. every lane gathers x for its values, multiplies and adds into the
-//      tile's per-row LDS accumulators (ds_add);
-// and at the end of the tile each row does one y[row] += sum.
+//   5. in value order, every lane gathers x for its values (neighbouring
+//      lanes read neighbouring columns) and forms the products;
+//   6. row sums, two ways (SPC5_REDUCE):
+//      1 (default) -- the products go to LDS slot-major, so that lane l then
+//        owns the kSlots consecutive values l*kSlots ...; it adds them in
+//        registers and does one LDS add per run of equal rows;
+//      0 -- DPP segmented sums over runs of equal rows inside each 16-lane
+//        row, one LDS add per run.
+// At the end of the tile each row is written once: y = alpha*sum + beta*y.
 #pragma once
 
 #include "spc5_mat.hpp"
 
 namespace spc5 {
 
-#ifndef SPC5_PIPELINE
-#define SPC5_PIPELINE 0
+#ifndef SPC5_REDUCE
+#define SPC5_REDUCE 1
 #endif
+#ifndef SPC5_PIPE
+#define SPC5_PIPE 1
+#endif
+
+template <class ACC>
+struct SlotLayout {
+    static constexpr int kSlots = kExpandCap / 64;  // values per lane in the row-run phase
+    // Slot-major product layout: value q = l*kSlots + k lives at k*kStride + l.
+    // The extra entries put the 64 lanes of one value-order store on
+    // distinct banks.
+    static constexpr int kStride = sizeof(ACC) == 8 ? 68 : 72;
+    __device__ __forceinline__ static int pos(int q) { return (q % kSlots) * kStride + q / kSlots; }
+};
 
 template <class ACC, int R>
 struct TileSmem {
-    ACC acc[kMaxRows];
+    ACC acc[kMaxRows];  // row sums of the tile
+#if SPC5_REDUCE
+    // Columns in value order while x is gathered; every column is read
+    // before the first product is stored over them.
+    union {
+        int col[kExpandCap];
+        ACC prod[SlotLayout<ACC>::kSlots * SlotLayout<ACC>::kStride];
+    };
+#else
+    int col[kExpandCap];
+#endif
+    alignas(8) unsigned char row[kExpandCap];  // tile-local row of batch value q
     int rs[kMaxRows / R + 1];
-    int col[1 + SPC5_PIPELINE][kExpandCap];
-    unsigned char row[1 + SPC5_PIPELINE][kExpandCap];
 };
 
 __device__ __forceinline__ int find_blockrow(const int* rs, int nbr, int top, int jr) {
@@ -58,6 +88,24 @@ __device__ __forceinline__ void load_header(const unsigned char* __restrict__ hd
     }
 }
 
+// Header of block min(j, last) with no branch around the loads, so that a
+// prefetch stays in flight until its first use (a load under a lane
+// condition is merged with the default right after it, which waits for
+// every older load too).  Lanes past the end are masked where the header is
+// used.
+template <int R, int MB>
+__device__ __forceinline__ void load_header_clamped(const unsigned char* __restrict__ hdr, int j, int last,
+                                                    int& cs, unsigned long long* mw) {
+    const int jj = j < last ? j : last;
+    const unsigned char* h = hdr + (int64_t)jj * (4 + R * MB);
+    cs = load_i32_unaligned(h);
+    if constexpr (R * MB == 1) {
+        mw[0] = (unsigned)h[4];  // the byte load zero-extends; nothing waits for it here
+    } else {
+        MaskWords<R * MB>::load(h + 4, mw);
+    }
+}
+
 struct BatchShape {
     int cnt;    // values of this lane's block
     int incl;   // inclusive prefix of cnt over the lanes
@@ -73,7 +121,7 @@ __device__ __forceinline__ BatchShape batch_shape(const unsigned long long* mw, 
     for (int k = 0; k < NW; ++k) bs.cnt += __popcll(mw[k]);
     bs.incl = wave_incl_scan(bs.cnt);
     bs.ntake = __popcll(ballot64(valid && bs.incl <= kExpandCap));
-    bs.total = __builtin_amdgcn_readlane(bs.incl, bs.ntake - 1);
+    bs.total = bs.ntake > 0 ? __builtin_amdgcn_readlane(bs.incl, bs.ntake - 1) : 0;
     return bs;
 }
 
@@ -86,22 +134,94 @@ __device__ __forceinline__ void load_values(const V* __restrict__ val, int v, in
     }
 }
 
-// Every set mask bit of the taken blocks -> (column, tile-local row) in LDS.
+// Values of a batch with no branch around the loads (a load under a lane
+// condition may not be issued, and waits for later loads then count only
+// the unconditional ones): lanes past the batch's end read value `last`,
+// whose products are never summed.
+template <class V, class ACC, int K>
+__device__ __forceinline__ void load_values_clamped(const V* __restrict__ val, int v, int last, int lane, ACC* a) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = v + lane + 64 * k;
+        a[k] = ValueLoad<V, ACC>::get(val + (i < last ? i : last));
+    }
+}
+
+// Every set mask bit of the taken blocks -> (column, tile-local row) in LDS,
+// starting at batch position p.
 template <int R, int VS, int NW>
 __device__ __forceinline__ void expand(int* col, unsigned char* row, const int* rs, int nbr, int top, int jr,
                                        int cs, const unsigned long long* mw, int p) {
     const int rbase = find_blockrow(rs, nbr, top, jr) * R;
+    if constexpr (R * VS <= 16) {
+        // Few mask bits: test each one (no data-dependent trip count).
+        const unsigned m = (unsigned)mw[0];
 #pragma unroll
-    for (int wd = 0; wd < NW; ++wd) {
-        unsigned long long m = mw[wd];
+        for (int bit = 0; bit < R * VS; ++bit) {
+            if ((m >> bit) & 1u) {
+                col[p] = cs + (bit % VS);
+                row[p] = (unsigned char)(rbase + bit / VS);
+                ++p;
+            }
+        }
+    } else if constexpr (R * VS <= 32) {
+        unsigned m = (unsigned)mw[0];
         while (m) {
-            const int bit = __builtin_ctzll(m) + wd * 64;
+            const int bit = __builtin_ctz(m);
             col[p] = cs + (bit & (VS - 1));
             row[p] = (unsigned char)(rbase + bit / VS);
             ++p;
-            m &= m - 1ull;
+            m &= m - 1u;
+        }
+    } else {
+#pragma unroll
+        for (int wd = 0; wd < NW; ++wd) {
+            unsigned long long m = mw[wd];
+            while (m) {
+                const int bit = __builtin_ctzll(m) + wd * 64;
+                col[p] = cs + (bit & (VS - 1));
+                row[p] = (unsigned char)(rbase + bit / VS);
+                ++p;
+                m &= m - 1ull;
+            }
         }
     }
+}
+
+// Row sums of one batch by lane-owned runs: lane l owns batch values
+// l*K .. l*K+K-1 (products in LDS, slot-major).  Consecutive values of one
+// row are added in registers; every run of equal rows ends with one LDS add.
+template <class ACC, int K>
+__device__ __forceinline__ void reduce_slots(ACC* acc, const unsigned char* row, const ACC* prod, int total,
+                                             int lane) {
+    using L = SlotLayout<ACC>;
+    constexpr int kNone = kMaxRows;  // row of the slots past the batch's end
+    ACC p[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) p[k] = prod[k * L::kStride + lane];  // all reads before the first add
+    int rw[K];
+    const int nv = total - lane * K;  // valid slots of this lane (<= 0: none, >= K: all)
+    if constexpr (K == 8) {
+        unsigned long long word;
+        __builtin_memcpy(&word, row + lane * 8, 8);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) rw[k] = k < nv ? (int)((word >> (8 * k)) & 0xffu) : kNone;
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) rw[k] = k < nv ? (int)row[lane * K + k] : kNone;
+    }
+    // Slots past the end form a suffix, so a run is only ever closed on a
+    // real row; the last run is dropped when it is past the end.
+    ACC s = p[0];
+    int rp = rw[0];
+#pragma unroll
+    for (int k = 1; k < K; ++k) {
+        const bool change = rw[k] != rp;
+        if (change) atomicAdd(&acc[rp], s);
+        s = change ? p[k] : s + p[k];
+        rp = rw[k];
+    }
+    if (rp != kNone) atomicAdd(&acc[rp], s);
 }
 
 template <class X, class ACC, int K>
@@ -120,182 +240,129 @@ __device__ __forceinline__ void gather_x(const int* col, const unsigned char* ro
 }
 
 template <class ACC, int K>
-__device__ __forceinline__ void accumulate(ACC* acc, int total, const ACC* a, const ACC* xv, const int* rw) {
+__device__ __forceinline__ void accumulate_dpp(ACC* acc, int total, const ACC* a, const ACC* xv, const int* rw) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (64 * k >= total) break;  // wave-uniform
         const bool ok = rw[k] >= 0;
-#if SPC5_SEGREDUCE
         // Runs of one row in adjacent lanes are summed first (DPP), so one
         // LDS atomic per run replaces one per value.
         ACC pr = ok ? a[k] * xv[k] : (ACC)0;
         const bool tail = row_segmented_sum(pr, rw[k]);
         if (ok && tail) atomicAdd(&acc[rw[k]], pr);
-#else
-        if (ok) atomicAdd(&acc[rw[k]], a[k] * xv[k]);
-#endif
     }
 }
 
-// ---------------------------------------------------------------------------
-// Block-stream tile (the default).  Per batch of up to 64 blocks:
-//   * values of the batch: coalesced loads from the stream into LDS;
-//   * each taken lane loads its block's x window x[start .. start+VS) (the
-//     reference's unaligned VS-wide x load) into its own LDS row;
-//   * each lane then walks its block's set mask bits in stream order (rows
-//     in order, columns ascending): sum += value * x_window[bit % VS] from
-//     LDS, and adds the sum to the tile's row accumulator when the row
-//     changes.
-// No per-value decode, gather or cross-lane reduction: the work per value is
-// two LDS reads and an FMA.
-// ---------------------------------------------------------------------------
-template <class X, class ACC, int R, int VS>
-struct StreamSmem {
-    static constexpr int kXStride = VS + 1;  // padded per-lane x window (bank spread)
-    ACC acc[kMaxRows];
-    int rs[kMaxRows / R + 1];
-    ACC val[kExpandCap];
-    X xw[64 * kXStride];
+// Raw header of one block in registers: the int32 first column and the r
+// masks as 32-bit words exactly as loaded (a byte or short load already
+// zero-extends), widened only where they are first used.
+template <int R, int MB>
+struct HeaderRegs {
+    static constexpr int kBytes = R * MB;
+    static constexpr int kWords = kBytes <= 4 ? 1 : kBytes / 4;
+    // A byte or short mask is read as the (unaligned) dword of the header
+    // that ends with it, and shifted down only in take_batch: an extension
+    // written next to the load would be scheduled right after it and wait.
+    int cs = 0;
+    unsigned m[kWords] = {};
 };
 
-template <class V, class X, class ACC, int R, int VS, int MB>
-__device__ __forceinline__ void spmv_tile_stream(StreamSmem<X, ACC, R, VS>& sm, const int* __restrict__ bptr,
-                                                 const unsigned char* __restrict__ hdr, const V* __restrict__ val,
-                                                 const X* __restrict__ x, X* __restrict__ y, int br0, int br1,
-                                                 int vstart, int64_t nrows, int64_t ncols, const Epilogue& ep) {
-    constexpr int K = kExpandCap / 64;
-    constexpr int NW = MaskWords<R * MB>::kWords;
-    constexpr int XS = StreamSmem<X, ACC, R, VS>::kXStride;
-    const int lane = lane_id();
-
-    const int nbr = br1 - br0;
-    const int b0 = bptr[br0];
-    const int b1 = bptr[br1];
-    int v = vstart;
-    const int64_t row0 = (int64_t)br0 * R;
-    const int64_t rows_left = nrows - row0;
-    const int nrow_t = (int)((int64_t)nbr * R < rows_left ? (int64_t)nbr * R : rows_left);
-
-    for (int i = lane; i <= nbr; i += 64) sm.rs[i] = bptr[br0 + i] - b0;
-    for (int i = lane; i < nbr * R; i += 64) sm.acc[i] = (ACC)0;
-    int top = 1;
-    while (top < nbr) top <<= 1;
-
-    int b = b0;
+template <int NW>
+struct BatchState {
+    BatchShape bs;
     int cs;
     unsigned long long mw[NW];
-    load_header<R, MB>(hdr, b + lane, b1, cs, mw);
-    wave_sync();
-    X* xw = &sm.xw[lane * XS];
+};
 
-    while (b < b1) {
-        const BatchShape bs = batch_shape<NW>(mw, b + lane < b1);
-        // values of the batch -> LDS (coalesced)
+// Header of block min(j, last) into raw registers, no branch around the
+// loads (see load_header_clamped).
+template <int R, int MB>
+__device__ __forceinline__ void load_header_raw(const unsigned char* __restrict__ hdr, int j, int last,
+                                                HeaderRegs<R, MB>& h) {
+    constexpr int B = R * MB;
+    const int jj = j < last ? j : last;
+    const unsigned char* p = hdr + (int64_t)jj * (4 + B);
+    h.cs = load_i32_unaligned(p);
+    if constexpr (B <= 2) {
+        __builtin_memcpy(&h.m[0], p + B, 4);  // header bytes B .. B+3; the mask is the top B bytes
+    } else {
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (64 * k >= bs.total) break;  // wave-uniform
-            const int q = lane + 64 * k;
-            if (q < bs.total) sm.val[q] = ValueLoad<V, ACC>::get(val + v + q);
-        }
-        const bool taken = lane < bs.ntake;
-        const int jcur = b + lane;
-        const int cs_cur = cs;
-        unsigned long long m_cur[NW];
-#pragma unroll
-        for (int k = 0; k < NW; ++k) m_cur[k] = mw[k];
-        // the block's x window -> LDS
-        if (taken) {
-            if ((int64_t)cs_cur + VS <= ncols) {
-                X w[VS];
-                __builtin_memcpy(w, x + cs_cur, sizeof(w));
-#pragma unroll
-                for (int j = 0; j < VS; ++j) xw[j] = w[j];
-            } else {
-#pragma unroll
-                for (int j = 0; j < VS; ++j) {
-                    const int64_t c = (int64_t)cs_cur + j < ncols ? (int64_t)cs_cur + j : ncols - 1;
-                    xw[j] = x[c];
-                }
-            }
-        }
-        const int bn = b + bs.ntake;
-        load_header<R, MB>(hdr, bn + lane, b1, cs, mw);
-        const int rbase = taken ? find_blockrow(sm.rs, nbr, top, jcur - b0) * R : 0;
-        wave_sync();
-        if (taken) {
-            int p = bs.incl - bs.cnt;
-            ACC sum = (ACC)0;
-            if constexpr (R == 1 && R * MB <= 4) {
-                unsigned m = (unsigned)m_cur[0];
-                while (m) {
-                    const int bit = __builtin_ctz(m);
-                    sum = __builtin_fma(sm.val[p], (ACC)xw[bit], sum);
-                    ++p;
-                    m &= m - 1u;
-                }
-                if (bs.cnt) atomicAdd(&sm.acc[rbase], sum);
-            } else if constexpr (R * MB <= 4) {
-                unsigned m = (unsigned)m_cur[0];
-                int cur = -1;
-                while (m) {
-                    const int bit = __builtin_ctz(m);
-                    const int s = bit / VS;
-                    if (s != cur) {
-                        if (cur >= 0) atomicAdd(&sm.acc[rbase + cur], sum);
-                        cur = s;
-                        sum = (ACC)0;
-                    }
-                    sum = __builtin_fma(sm.val[p], (ACC)xw[bit & (VS - 1)], sum);
-                    ++p;
-                    m &= m - 1u;
-                }
-                if (cur >= 0) atomicAdd(&sm.acc[rbase + cur], sum);
-            } else {
-                int cur = -1;
-#pragma unroll
-                for (int wd = 0; wd < NW; ++wd) {
-                    unsigned long long m = m_cur[wd];
-                    while (m) {
-                        const int bit = __builtin_ctzll(m) + wd * 64;
-                        const int s = bit / VS;
-                        if (s != cur) {
-                            if (cur >= 0) atomicAdd(&sm.acc[rbase + cur], sum);
-                            cur = s;
-                            sum = (ACC)0;
-                        }
-                        sum = __builtin_fma(sm.val[p], (ACC)xw[bit & (VS - 1)], sum);
-                        ++p;
-                        m &= m - 1ull;
-                    }
-                }
-                if (cur >= 0) atomicAdd(&sm.acc[rbase + cur], sum);
-            }
-        }
-        v += bs.total;
-        b = bn;
-        wave_sync();
+        for (int k = 0; k < B / 4; ++k) __builtin_memcpy(&h.m[k], p + 4 + 4 * k, 4);
     }
+}
+
+// Shape of the batch starting at block b from its (prefetched) headers;
+// lanes past the tile's last block count as empty.  This is the first use
+// of the header registers, so their wait happens here.
+template <int R, int MB, int NW>
+__device__ __forceinline__ BatchState<NW> take_batch(const HeaderRegs<R, MB>& h, int b, int b1, int lane) {
+    using H = HeaderRegs<R, MB>;
+    BatchState<NW> st;
+    // The empty asm reads the header registers: it keeps their first use
+    // (and so the wait for them) at this point, after earlier requests.
+    int cs = h.cs;
+    unsigned m[H::kWords];
+    if constexpr (H::kBytes == 1) {
+        asm volatile("v_lshrrev_b32 %0, 24, %1" : "=v"(m[0]) : "v"(h.m[0]));
+    } else if constexpr (H::kBytes == 2) {
+        asm volatile("v_lshrrev_b32 %0, 16, %1" : "=v"(m[0]) : "v"(h.m[0]));
+    } else {
+#pragma unroll
+        for (int k = 0; k < H::kWords; ++k) {
+            m[k] = h.m[k];
+            asm volatile("" : "+v"(m[k]));
+        }
+    }
+    asm volatile("" : "+v"(cs));
+    const bool valid = b + lane < b1;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        unsigned long long w = 0;
+        if constexpr (H::kWords == 1) {
+            w = m[0];
+        } else {
+            w = (unsigned long long)m[2 * k] | ((unsigned long long)m[2 * k + 1] << 32);
+        }
+        st.mw[k] = valid ? w : 0ull;
+    }
+    st.cs = valid ? cs : 0;
+    st.bs = batch_shape<NW>(st.mw, valid);
+    return st;
+}
+
+// x for every value of one expanded batch: columns of every chunk first
+// (past the batch's end: column 0, a valid x entry whose product with the
+// zero value is unused), then all K requests.  No branch, so that nothing
+// waits between the requests.
+template <class X, class ACC, int K, class SM>
+__device__ __forceinline__ void request_x(SM& sm, const X* __restrict__ x, int total, int lane, ACC* xv) {
+    int cl[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int q = lane + 64 * k;
+        const int c = sm.col[q];
+        cl[k] = q < total ? c : 0;
+    }
+    wave_sync();  // every column is read before products overwrite them
+#pragma unroll
+    for (int k = 0; k < K; ++k) xv[k] = (ACC)x[cl[k]];
+}
+
+// Products of one batch (values a, x xv) and their row sums into the
+// tile's accumulators.  Chunks past the batch's end store unused products.
+template <class ACC, int K, class SM>
+__device__ __forceinline__ void reduce_batch(SM& sm, const ACC* a, const ACC* xv, int total, int lane) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sm.prod[SlotLayout<ACC>::pos(lane + 64 * k)] = a[k] * xv[k];
     wave_sync();
-    const ACC alpha = (ACC)ep.alpha, beta = (ACC)ep.beta;
-    const X* __restrict__ dv = reinterpret_cast<const X*>(ep.dv);
-    double part = 0.0;
-    for (int i = lane; i < nrow_t; i += 64) {
-        const ACC ax = sm.acc[i];
-        const X yn = ep.beta == 0.0 ? (X)(alpha * ax) : (X)(alpha * ax + beta * (ACC)y[row0 + i]);
-        y[row0 + i] = yn;
-        if (dv) part += (double)dv[row0 + i] * (double)yn;
-    }
-    if (ep.dot) {
-        part = wave_sum_f64(part);
-        if (lane == 0) ep.dot[0] = part;
-    }
+    reduce_slots<ACC, K>(sm.acc, sm.row, sm.prod, total, lane);
 }
 
 template <class V, class X, class ACC, int R, int VS, int MB>
 __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __restrict__ bptr,
                                           const unsigned char* __restrict__ hdr, const V* __restrict__ val,
                                           const X* __restrict__ x, X* __restrict__ y, int br0, int br1, int vstart,
-                                          int64_t nrows, const Epilogue& ep) {
+                                          int vend, int64_t nrows, const Epilogue& ep) {
     constexpr int K = kExpandCap / 64;
     constexpr int NW = MaskWords<R * MB>::kWords;
     const int lane = lane_id();
@@ -313,72 +380,84 @@ __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __res
     int top = 1;
     while (top < nbr) top <<= 1;
 
-    int b = b0;
-    int cs;
-    unsigned long long mw[NW];
-    load_header<R, MB>(hdr, b + lane, b1, cs, mw);
-    wave_sync();
-
-#if SPC5_PIPELINE
-    // Two batches in flight: batch i's x gathers travel while batch i+1 is
-    // scanned, its values requested and its blocks expanded into the other
-    // LDS buffer.
-    ACC a_cur[K], xv_cur[K];
-    int rw_cur[K];
-    int total_cur = 0;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        a_cur[k] = (ACC)0;
-        xv_cur[k] = (ACC)0;
-        rw_cur[k] = -1;
+#if SPC5_PIPE && SPC5_REDUCE
+    // Software pipeline over batches: while batch i is expanded and reduced,
+    // the values of batch i+1 and the headers of batch i+2 are in flight.
+    // Two register sets (values, headers) alternate between the halves of
+    // the unrolled loop, so no in-flight register is ever copied.
+    // A tile without values never touches x (x may be null when the matrix
+    // has no values); inside a tile with values x has at least one entry.
+    if (b0 < b1 && vstart < vend) {
+        int b = b0;
+        HeaderRegs<R, MB> h0, h1;
+        load_header_raw<R, MB>(hdr, b + lane, b1 - 1, h0);
+        BatchState<NW> cur = take_batch<R, MB, NW>(h0, b, b1, lane);
+        ACC a0[K], a1[K];
+        load_values_clamped<V, ACC, K>(val, v, vend - 1, lane, a0);
+        int bn = b + cur.bs.ntake;
+        load_header_raw<R, MB>(hdr, bn + lane, b1 - 1, h1);
+        auto step = [&](ACC(&acur)[K], ACC(&anxt)[K], HeaderRegs<R, MB>& hcons, HeaderRegs<R, MB>& hload) {
+            // expand batch i and request its x ...
+            if (lane < cur.bs.ntake)
+                expand<R, VS, NW>(sm.col, sm.row, sm.rs, nbr, top, b + lane - b0, cur.cs, cur.mw,
+                                  cur.bs.incl - cur.bs.cnt);
+            wave_sync();
+            ACC xv[K];
+            request_x<X, ACC, K>(sm, x, cur.bs.total, lane, xv);
+            compiler_fence();
+            // ... then shape batch i+1 and request its values and the headers
+            // of batch i+2: these are younger than batch i's x requests, so
+            // the products below do not wait for them.
+            const BatchState<NW> nxt = take_batch<R, MB, NW>(hcons, bn, b1, lane);
+            load_values_clamped<V, ACC, K>(val, v + cur.bs.total, vend - 1, lane, anxt);
+            const int bnn = bn + nxt.bs.ntake;
+            load_header_raw<R, MB>(hdr, bnn + lane, b1 - 1, hload);  // past the end: masked
+            compiler_fence();
+            reduce_batch<ACC, K>(sm, acur, xv, cur.bs.total, lane);
+            v += cur.bs.total;
+            b = bn;
+            bn = bnn;
+            cur = nxt;
+            wave_sync();
+        };
+        for (;;) {
+            step(a0, a1, h1, h0);
+            if (b >= b1) break;
+            step(a1, a0, h0, h1);
+            if (b >= b1) break;
+        }
     }
-    int buf = 0;
-    while (b < b1) {
-        const BatchShape bs = batch_shape<NW>(mw, b + lane < b1);
-        ACC a_nxt[K];
-        load_values<V, ACC, K>(val, v, bs.total, lane, a_nxt);
-        const int jcur = b + lane;
-        const int cs_cur = cs;
-        unsigned long long mw_cur[NW];
-#pragma unroll
-        for (int k = 0; k < NW; ++k) mw_cur[k] = mw[k];
-        const int bn = b + bs.ntake;
-        load_header<R, MB>(hdr, bn + lane, b1, cs, mw);
-        if (lane < bs.ntake)
-            expand<R, VS, NW>(sm.col[buf], sm.row[buf], sm.rs, nbr, top, jcur - b0, cs_cur, mw_cur,
-                              bs.incl - bs.cnt);
-        wave_sync();
-        accumulate<ACC, K>(sm.acc, total_cur, a_cur, xv_cur, rw_cur);
-        gather_x<X, ACC, K>(sm.col[buf], sm.row[buf], x, bs.total, lane, xv_cur, rw_cur);
-#pragma unroll
-        for (int k = 0; k < K; ++k) a_cur[k] = a_nxt[k];
-        total_cur = bs.total;
-        v += bs.total;
-        b = bn;
-        buf ^= 1;
-        wave_sync();
-    }
-    accumulate<ACC, K>(sm.acc, total_cur, a_cur, xv_cur, rw_cur);
 #else
+    int b = b0;
+    HeaderRegs<R, MB> h;
+    if (vstart == vend) b = b1;  // no values: x is never read
+    if (b < b1) load_header_raw<R, MB>(hdr, b + lane, b1 - 1, h);
+    wave_sync();
     while (b < b1) {
-        const BatchShape bs = batch_shape<NW>(mw, b + lane < b1);
+        const BatchState<NW> cur = take_batch<R, MB, NW>(h, b, b1, lane);
         ACC a[K];
-        load_values<V, ACC, K>(val, v, bs.total, lane, a);
-        const int jcur = b + lane;
-        const int cs_cur = cs;
-        unsigned long long mw_cur[NW];
-#pragma unroll
-        for (int k = 0; k < NW; ++k) mw_cur[k] = mw[k];
-        const int bn = b + bs.ntake;
-        load_header<R, MB>(hdr, bn + lane, b1, cs, mw);
-        if (lane < bs.ntake)
-            expand<R, VS, NW>(sm.col[0], sm.row[0], sm.rs, nbr, top, jcur - b0, cs_cur, mw_cur, bs.incl - bs.cnt);
+        load_values<V, ACC, K>(val, v, cur.bs.total, lane, a);
+        const int bn = b + cur.bs.ntake;
+        load_header_raw<R, MB>(hdr, bn + lane, b1 - 1, h);  // past the end: block b1-1, masked
+        if (lane < cur.bs.ntake)
+            expand<R, VS, NW>(sm.col, sm.row, sm.rs, nbr, top, b + lane - b0, cur.cs, cur.mw,
+                              cur.bs.incl - cur.bs.cnt);
         wave_sync();
-        ACC xv[K];
-        int rw[K];
-        gather_x<X, ACC, K>(sm.col[0], sm.row[0], x, bs.total, lane, xv, rw);
-        accumulate<ACC, K>(sm.acc, bs.total, a, xv, rw);
-        v += bs.total;
+#if SPC5_REDUCE
+        if (cur.bs.total > 0) {
+            ACC xv[K];
+            request_x<X, ACC, K>(sm, x, cur.bs.total, lane, xv);
+            reduce_batch<ACC, K>(sm, a, xv, cur.bs.total, lane);
+        }
+#else
+        {
+            ACC xv[K];
+            int rw[K];
+            gather_x<X, ACC, K>(sm.col, sm.row, x, cur.bs.total, lane, xv, rw);
+            accumulate_dpp<ACC, K>(sm.acc, cur.bs.total, a, xv, rw);
+        }
+#endif
+        v += cur.bs.total;
         b = bn;
         wave_sync();
     }

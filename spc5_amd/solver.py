@@ -3,7 +3,8 @@ CG / mixed-precision iterative-refinement solvers of libspc5_hip.so.
 
     A64 = DeviceMatrix.from_csr(rowptr, col, values_f64, r=4)     # beta(4,8)
     A16 = lowp(A64, "bf16")      # same headers, bfloat16 values
-    x, st = ir_cg(A64, A16, b, rtol=1e-10)
+    x, st = ir_cg(A64, A16, b, rtol=1e-10)       # restarted refinement
+    x, st = mixed_cg(A64, A16, b, rtol=1e-10)    # reliable updates
 """
 from __future__ import annotations
 
@@ -64,4 +65,20 @@ def ir_cg(A64: DeviceMatrix, Alo: DeviceMatrix, b, x=None, rtol=1e-10, inner_rto
     st = SolveStats()
     check(lib().spc5_hip_ir_cg(A64._h, Alo._h, _ptr(b), _ptr(x), float(rtol), float(inner_rtol), int(max_outer),
                                int(max_inner), C.byref(st), stream_ptr(stream)), "spc5_hip_ir_cg")
+    return x, {"iterations": st.iterations, "outer": st.outer, "relres": st.relres}
+
+
+def mixed_cg(A64: DeviceMatrix, Alo: DeviceMatrix, b, x=None, rtol=1e-10, delta=None, maxit=20000, stream=None):
+    """CG with reliable updates: float iteration on Alo, double solution and
+    residual (b - A64 x recomputed whenever the float residual has dropped by
+    `delta`, default 0.1 for float values and 0.5 for bfloat16)."""
+    _need_cuda(b, "b", torch.float64)
+    if x is None:
+        x = torch.zeros_like(b)
+    _need_cuda(x, "x", torch.float64)
+    if delta is None:
+        delta = 0.5 if Alo.values.dtype == torch.bfloat16 else 0.1
+    st = SolveStats()
+    check(lib().spc5_hip_mixed_cg(A64._h, Alo._h, _ptr(b), _ptr(x), float(rtol), float(delta), int(maxit),
+                                  C.byref(st), stream_ptr(stream)), "spc5_hip_mixed_cg")
     return x, {"iterations": st.iterations, "outer": st.outer, "relres": st.relres}

@@ -124,6 +124,53 @@ def test_ir_cg_mixed(kind, r):
     assert np.linalg.norm(xh - xs) / np.linalg.norm(xs) <= 1e-7
 
 
+@pytest.mark.parametrize("kind", ["f32", "bf16"])
+@pytest.mark.parametrize("r", (1, 4))
+def test_mixed_cg_reliable_updates(kind, r):
+    """Reliable-update CG: float iteration, double residual/solution, the
+    true residual below 1e-10 and the solution equal to a dense solve."""
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import lowp, mixed_cg
+    rp, col, val = stencil27_small(12)
+    n = len(rp) - 1
+    A64 = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val), r)
+    Alo = lowp(A64, kind)
+    b = np.random.default_rng(6).uniform(-1, 1, n)
+    x, st = mixed_cg(A64, Alo, dev(b), rtol=1e-10)
+    xh = x.cpu().numpy()
+    Ad = csr_dense(rp, col, val, n)
+    relres = np.linalg.norm(b - Ad @ xh) / np.linalg.norm(b)
+    assert relres <= 1e-10, (relres, st)
+    assert st["relres"] == pytest.approx(relres, rel=1e-2, abs=1e-13)
+    assert st["outer"] >= 2  # at least one intermediate update besides the last
+    xs = np.linalg.solve(Ad, b)
+    assert np.linalg.norm(xh - xs) / np.linalg.norm(xs) <= 1e-7
+
+
+def test_mixed_cg_nonzero_guess_and_limits():
+    """A nonzero initial guess is refined, maxit=0 leaves x alone, bad
+    arguments are rejected."""
+    from spc5_amd._lib import Spc5Error
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import lowp, mixed_cg
+    rp, col, val = stencil27_small(8)
+    n = len(rp) - 1
+    A64 = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val), 2)
+    Alo = lowp(A64, "f32")
+    b = np.random.default_rng(8).uniform(-1, 1, n)
+    x0 = np.random.default_rng(9).uniform(-1, 1, n)
+    x = dev(x0.copy())
+    x, st = mixed_cg(A64, Alo, dev(b), x=x, rtol=1e-10, maxit=0)
+    assert np.array_equal(x.cpu().numpy(), x0) and st["iterations"] == 0
+    x, st = mixed_cg(A64, Alo, dev(b), x=x, rtol=1e-10)
+    Ad = csr_dense(rp, col, val, n)
+    assert np.linalg.norm(b - Ad @ x.cpu().numpy()) / np.linalg.norm(b) <= 1e-10
+    with pytest.raises(Spc5Error):
+        mixed_cg(A64, Alo, dev(b), rtol=1e-10, delta=1.5)
+    with pytest.raises(Spc5Error):
+        mixed_cg(A64, A64, dev(b), rtol=1e-10)
+
+
 def test_cg_float_reference_format():
     """Float CG on the reference float format (VS=16): converges to 1e-5."""
     from spc5_amd.device import DeviceMatrix
