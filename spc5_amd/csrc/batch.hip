@@ -109,11 +109,11 @@ int launch_batch_any(const spc5_hip_batch& b, hipStream_t s) {
     if (b.value_type == SPC5_HIP_F64)
         launch_batch<double, double, double, 8, 1>(b, s);
     else if (b.value_type == SPC5_HIP_F32 && b.vs == 16)
-        launch_batch<float, float, float, 16, 2>(b, s);
+        launch_batch<float, float, FloatAcc, 16, 2>(b, s);
     else if (b.value_type == SPC5_HIP_F32)
-        launch_batch<float, float, float, 8, 1>(b, s);
+        launch_batch<float, float, FloatAcc, 8, 1>(b, s);
     else
-        launch_batch<unsigned short, float, float, 8, 1>(b, s);
+        launch_batch<unsigned short, float, FloatAcc, 8, 1>(b, s);
     SPC5_HIP_TRY(hipGetLastError());
     return SPC5_HIP_OK;
 }

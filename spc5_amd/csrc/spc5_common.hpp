@@ -81,6 +81,20 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     return v;
 }
 
+// Inclusive prefix maximum of a non-negative int over the 64 lanes (same
+// structure as wave_incl_scan; lanes outside a row read 0).
+__device__ __forceinline__ int wave_incl_max(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, kDppRowShr1, 0xf, 0xf, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, kDppRowShr2, 0xf, 0xf, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, kDppRowShr4, 0xf, 0xf, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, kDppRowShr8, 0xf, 0xf, true));
+    const int c1 = __builtin_amdgcn_readlane(v, 15);
+    const int c2 = max(c1, __builtin_amdgcn_readlane(v, 31));
+    const int c3 = max(c2, __builtin_amdgcn_readlane(v, 47));
+    const int row = lane_id() >> 4;
+    return max(v, row == 0 ? 0 : (row == 1 ? c1 : (row == 2 ? c2 : c3)));
+}
+
 // DPP moves of 32/64-bit values inside 16-lane rows (0 where the source
 // lane is outside the row).
 constexpr int kDppRowShl1 = 0x101;
@@ -221,6 +235,12 @@ struct ValueLoad {
 template <>
 struct ValueLoad<unsigned short, float> {
     __device__ __forceinline__ static float get(const unsigned short* p) { return bf16_to_f32(load_stream(p)); }
+};
+template <>
+struct ValueLoad<unsigned short, double> {
+    __device__ __forceinline__ static double get(const unsigned short* p) {
+        return (double)bf16_to_f32(load_stream(p));
+    }
 };
 
 // --------------------------------------------------------------------------

@@ -28,6 +28,16 @@ namespace spc5 {
 #ifndef SPC5_XCD_REMAP
 #define SPC5_XCD_REMAP 1
 #endif
+// Accumulator type of the float and bfloat16 kernels' LDS row sums (the
+// products are formed in it too): SPC5_FLOAT_ACC_DOUBLE = 1 selects double.
+#ifndef SPC5_FLOAT_ACC_DOUBLE
+#define SPC5_FLOAT_ACC_DOUBLE 0
+#endif
+#if SPC5_FLOAT_ACC_DOUBLE
+using FloatAcc = double;
+#else
+using FloatAcc = float;
+#endif
 constexpr int kMaxRows = SPC5_MAX_ROWS;              // scalar rows per tile (LDS accumulators), <= 256
 constexpr int kWavesPerGroup = SPC5_WAVES_PER_GROUP; // independent waves per workgroup
 constexpr int kExpandCap = SPC5_EXPAND_CAP;          // values expanded into LDS per batch

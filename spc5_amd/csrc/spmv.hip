@@ -43,8 +43,14 @@ __device__ __forceinline__ int64_t xcd_tile(int wave) {
     return gx * kWavesPerGroup + wave;
 }
 
+// SPC5_WAVES_PER_EU (1: compiler's choice) asks for at least that many
+// waves per SIMD, i.e. a register budget of 512 / waves VGPRs.
+#ifndef SPC5_WAVES_PER_EU
+#define SPC5_WAVES_PER_EU 1
+#endif
+
 template <class V, class X, class ACC, int R, int VS, int MB>
-__global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_tiles(
+__global__ __launch_bounds__(kWavesPerGroup * 64, SPC5_WAVES_PER_EU) void k_spmv_tiles(
     const int* __restrict__ bptr, const unsigned char* __restrict__ hdr, const V* __restrict__ val,
     const X* __restrict__ x, X* __restrict__ y, const int* __restrict__ tile_br,
     const int* __restrict__ tile_val, int64_t ntiles, int64_t nrows, int64_t ncols, Epilogue ep) {
@@ -193,9 +199,9 @@ const char* last_error_cstr() {
 int launch_spmv(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s, const Epilogue& ep) {
     if (m.plan.ntiles == 0) return SPC5_HIP_OK;
     if (m.value_type == SPC5_HIP_F64) return launch_spmv_vs<double, double, double, 8, 1>(m, x, y, s, ep);
-    if (m.value_type == SPC5_HIP_F32 && m.vs == 16) return launch_spmv_vs<float, float, float, 16, 2>(m, x, y, s, ep);
-    if (m.value_type == SPC5_HIP_F32 && m.vs == 8) return launch_spmv_vs<float, float, float, 8, 1>(m, x, y, s, ep);
-    if (m.value_type == SPC5_HIP_BF16) return launch_spmv_vs<unsigned short, float, float, 8, 1>(m, x, y, s, ep);
+    if (m.value_type == SPC5_HIP_F32 && m.vs == 16) return launch_spmv_vs<float, float, FloatAcc, 16, 2>(m, x, y, s, ep);
+    if (m.value_type == SPC5_HIP_F32 && m.vs == 8) return launch_spmv_vs<float, float, FloatAcc, 8, 1>(m, x, y, s, ep);
+    if (m.value_type == SPC5_HIP_BF16) return launch_spmv_vs<unsigned short, float, FloatAcc, 8, 1>(m, x, y, s, ep);
     set_error("unsupported value type / VS");
     return SPC5_HIP_ERR_INVALID;
 }

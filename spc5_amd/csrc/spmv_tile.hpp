@@ -8,11 +8,11 @@
 //      kExpandCap values;
 //   2. the batch's values are loaded at once, kExpandCap/64 coalesced loads
 //      per lane, straight from the value stream (no zero padding);
-//   3. later headers are requested (SPC5_PIPE: steps 1-3 run one batch
-//      ahead, so a batch's values are in flight while the previous batch
-//      goes through steps 4-6);
-//   4. each taken lane finds its block-row (binary search over the tile's
-//      block-row pointers in LDS) and writes, for every set mask bit, the
+//   3. the next batch's headers are requested;
+//   4. the block-row of every block of the batch comes from one pass over
+//      the tile's block-row pointers in LDS (the block-rows that start in
+//      the batch mark their first block, a wave prefix maximum spreads the
+//      marks); each taken lane then writes, for every set mask bit, the
 //      column and the tile-local row into LDS;
 //   5. in value order, every lane gathers x for its values (neighbouring
 //      lanes read neighbouring columns) and forms the products;
@@ -31,9 +31,6 @@ namespace spc5 {
 
 #ifndef SPC5_REDUCE
 #define SPC5_REDUCE 1
-#endif
-#ifndef SPC5_PIPE
-#define SPC5_PIPE 1
 #endif
 
 template <class ACC>
@@ -61,6 +58,7 @@ struct TileSmem {
 #endif
     alignas(8) unsigned char row[kExpandCap];  // tile-local row of batch value q
     int rs[kMaxRows / R + 1];
+    int mark[64];  // block-row marks of one batch
 };
 
 __device__ __forceinline__ int find_blockrow(const int* rs, int nbr, int top, int jr) {
@@ -134,25 +132,12 @@ __device__ __forceinline__ void load_values(const V* __restrict__ val, int v, in
     }
 }
 
-// Values of a batch with no branch around the loads (a load under a lane
-// condition may not be issued, and waits for later loads then count only
-// the unconditional ones): lanes past the batch's end read value `last`,
-// whose products are never summed.
-template <class V, class ACC, int K>
-__device__ __forceinline__ void load_values_clamped(const V* __restrict__ val, int v, int last, int lane, ACC* a) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int i = v + lane + 64 * k;
-        a[k] = ValueLoad<V, ACC>::get(val + (i < last ? i : last));
-    }
-}
-
-// Every set mask bit of the taken blocks -> (column, tile-local row) in LDS,
-// starting at batch position p.
+// Every set mask bit of one block -> (column, tile-local row) in LDS,
+// starting at batch position p; rbase is the tile-local first row of the
+// block's block-row.
 template <int R, int VS, int NW>
-__device__ __forceinline__ void expand(int* col, unsigned char* row, const int* rs, int nbr, int top, int jr,
-                                       int cs, const unsigned long long* mw, int p) {
-    const int rbase = find_blockrow(rs, nbr, top, jr) * R;
+__device__ __forceinline__ void expand_rows(int* col, unsigned char* row, int rbase, int cs,
+                                            const unsigned long long* mw, int p) {
     if constexpr (R * VS <= 16) {
         // Few mask bits: test each one (no data-dependent trip count).
         const unsigned m = (unsigned)mw[0];
@@ -186,6 +171,42 @@ __device__ __forceinline__ void expand(int* col, unsigned char* row, const int* 
             }
         }
     }
+}
+
+template <int R, int VS, int NW>
+__device__ __forceinline__ void expand(int* col, unsigned char* row, const int* rs, int nbr, int top, int jr,
+                                       int cs, const unsigned long long* mw, int p) {
+    expand_rows<R, VS, NW>(col, row, find_blockrow(rs, nbr, top, jr) * R, cs, mw, p);
+}
+
+// Block-row of block `bb + lane` (tile-local block indices) for a batch of
+// `ntake` blocks, given `prev`, the block-row of block bb-1 (-1 when bb is
+// the tile's first block).  rs holds the tile's block-row pointers (nbr+1
+// entries, rs[0] = 0), mark is a 64-entry LDS scratch row.  The block-rows
+// prev+1 .. prev+128 are looked at (two per lane); those starting inside the
+// batch mark their first block with their rank, and a wave prefix maximum
+// gives every block the last block-row started at or before it.  When more
+// than 127 block-rows start inside the batch (runs of empty block-rows),
+// the lanes fall back to a binary search.
+__device__ __forceinline__ int batch_blockrows(const int* rs, int* mark, int nbr, int top, int bb, int ntake,
+                                               int prev, int lane) {
+    mark[lane] = 0;
+    const int i0 = prev + 1 + lane, i1 = i0 + 64;
+    const int s0 = i0 <= nbr ? rs[i0] : 0x7fffffff;
+    const int s1 = i1 <= nbr ? rs[i1] : 0x7fffffff;
+    wave_sync();
+    const int d0 = s0 - bb, d1 = s1 - bb;  // >= 0: block-row prev+1 starts at bb or later
+    if (d0 < ntake) atomicMax(&mark[d0], lane + 1);
+    if (d1 < ntake) atomicMax(&mark[d1], lane + 65);
+    const bool overflow = __builtin_amdgcn_readlane(d1, 63) < ntake;  // row prev+128 starts inside
+    wave_sync();
+    int br;
+    if (!overflow) {
+        br = prev + wave_incl_max(mark[lane]);
+    } else {
+        br = find_blockrow(rs, nbr, top, bb + lane);
+    }
+    return br;
 }
 
 // Row sums of one batch by lane-owned runs: lane l owns batch values
@@ -380,55 +401,8 @@ __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __res
     int top = 1;
     while (top < nbr) top <<= 1;
 
-#if SPC5_PIPE && SPC5_REDUCE
-    // Software pipeline over batches: while batch i is expanded and reduced,
-    // the values of batch i+1 and the headers of batch i+2 are in flight.
-    // Two register sets (values, headers) alternate between the halves of
-    // the unrolled loop, so no in-flight register is ever copied.
-    // A tile without values never touches x (x may be null when the matrix
-    // has no values); inside a tile with values x has at least one entry.
-    if (b0 < b1 && vstart < vend) {
-        int b = b0;
-        HeaderRegs<R, MB> h0, h1;
-        load_header_raw<R, MB>(hdr, b + lane, b1 - 1, h0);
-        BatchState<NW> cur = take_batch<R, MB, NW>(h0, b, b1, lane);
-        ACC a0[K], a1[K];
-        load_values_clamped<V, ACC, K>(val, v, vend - 1, lane, a0);
-        int bn = b + cur.bs.ntake;
-        load_header_raw<R, MB>(hdr, bn + lane, b1 - 1, h1);
-        auto step = [&](ACC(&acur)[K], ACC(&anxt)[K], HeaderRegs<R, MB>& hcons, HeaderRegs<R, MB>& hload) {
-            // expand batch i and request its x ...
-            if (lane < cur.bs.ntake)
-                expand<R, VS, NW>(sm.col, sm.row, sm.rs, nbr, top, b + lane - b0, cur.cs, cur.mw,
-                                  cur.bs.incl - cur.bs.cnt);
-            wave_sync();
-            ACC xv[K];
-            request_x<X, ACC, K>(sm, x, cur.bs.total, lane, xv);
-            compiler_fence();
-            // ... then shape batch i+1 and request its values and the headers
-            // of batch i+2: these are younger than batch i's x requests, so
-            // the products below do not wait for them.
-            const BatchState<NW> nxt = take_batch<R, MB, NW>(hcons, bn, b1, lane);
-            load_values_clamped<V, ACC, K>(val, v + cur.bs.total, vend - 1, lane, anxt);
-            const int bnn = bn + nxt.bs.ntake;
-            load_header_raw<R, MB>(hdr, bnn + lane, b1 - 1, hload);  // past the end: masked
-            compiler_fence();
-            reduce_batch<ACC, K>(sm, acur, xv, cur.bs.total, lane);
-            v += cur.bs.total;
-            b = bn;
-            bn = bnn;
-            cur = nxt;
-            wave_sync();
-        };
-        for (;;) {
-            step(a0, a1, h1, h0);
-            if (b >= b1) break;
-            step(a1, a0, h0, h1);
-            if (b >= b1) break;
-        }
-    }
-#else
     int b = b0;
+    int prev = -1;  // block-row of block b-1
     HeaderRegs<R, MB> h;
     if (vstart == vend) b = b1;  // no values: x is never read
     if (b < b1) load_header_raw<R, MB>(hdr, b + lane, b1 - 1, h);
@@ -439,9 +413,9 @@ __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __res
         load_values<V, ACC, K>(val, v, cur.bs.total, lane, a);
         const int bn = b + cur.bs.ntake;
         load_header_raw<R, MB>(hdr, bn + lane, b1 - 1, h);  // past the end: block b1-1, masked
-        if (lane < cur.bs.ntake)
-            expand<R, VS, NW>(sm.col, sm.row, sm.rs, nbr, top, b + lane - b0, cur.cs, cur.mw,
-                              cur.bs.incl - cur.bs.cnt);
+        const int br = batch_blockrows(sm.rs, sm.mark, nbr, top, b - b0, cur.bs.ntake, prev, lane);
+        prev = __builtin_amdgcn_readlane(br, cur.bs.ntake - 1);
+        if (lane < cur.bs.ntake) expand_rows<R, VS, NW>(sm.col, sm.row, br * R, cur.cs, cur.mw, cur.bs.incl - cur.bs.cnt);
         wave_sync();
 #if SPC5_REDUCE
         if (cur.bs.total > 0) {
@@ -461,7 +435,6 @@ __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __res
         b = bn;
         wave_sync();
     }
-#endif
     wave_sync();
     // y = alpha * (A x) + beta * y; y is not read when beta == 0.  With a dot
     // vector, the tile writes sum(dv[i] * y_new[i]) over its rows to *ep.dot.
