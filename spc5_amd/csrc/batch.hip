@@ -61,7 +61,7 @@ union BatchSmem {
     TileSmem<ACC, 8> r8;
 };
 #define SPC5_BATCH_TILE(RR, SMEM) \
-    spmv_tile<V, X, ACC, RR, VS, MB>(SMEM, m.bptr, m.hdr, val, x, y, br0, br1, v0, v1, m.nrows, ep)
+    spmv_tile<V, X, ACC, RR, VS, MB>(SMEM, m.bptr, m.hdr, val, x, y, br0, br1, v0, v1, m.nrows, m.ncols, ep)
 
 template <class V, class X, class ACC, int VS, int MB>
 __global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_batch(const BatchEntry* __restrict__ mats,

@@ -20,7 +20,7 @@ namespace spc5 {
 #define SPC5_WAVES_PER_GROUP 1
 #endif
 #ifndef SPC5_EXPAND_CAP
-#define SPC5_EXPAND_CAP 512
+#define SPC5_EXPAND_CAP 384
 #endif
 #ifndef SPC5_SEGREDUCE
 #define SPC5_SEGREDUCE 1

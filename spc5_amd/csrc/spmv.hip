@@ -60,9 +60,8 @@ __global__ __launch_bounds__(kWavesPerGroup * 64, SPC5_WAVES_PER_EU) void k_spmv
     if (t >= ntiles) return;  // whole wave leaves; no workgroup barrier is used
     Epilogue e = ep;
     if (e.dot) e.dot += t;  // one partial per tile
-    (void)ncols;
     spmv_tile<V, X, ACC, R, VS, MB>(sm[w], bptr, hdr, val, x, y, tile_br[t], tile_br[t + 1], tile_val[t],
-                                    tile_val[t + 1], nrows, e);
+                                    tile_val[t + 1], nrows, ncols, e);
 }
 
 // ---- plan kernels --------------------------------------------------------

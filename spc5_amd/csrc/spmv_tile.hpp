@@ -33,6 +33,7 @@ namespace spc5 {
 #define SPC5_REDUCE 1
 #endif
 
+
 template <class ACC>
 struct SlotLayout {
     static constexpr int kSlots = kExpandCap / 64;  // values per lane in the row-run phase
@@ -46,16 +47,14 @@ struct SlotLayout {
 template <class ACC, int R>
 struct TileSmem {
     ACC acc[kMaxRows];  // row sums of the tile
-#if SPC5_REDUCE
-    // Columns in value order while x is gathered; every column is read
-    // before the first product is stored over them.
+    // Columns in value order; every column is read before the first
+    // product is stored over them.
     union {
         int col[kExpandCap];
+#if SPC5_REDUCE
         ACC prod[SlotLayout<ACC>::kSlots * SlotLayout<ACC>::kStride];
-    };
-#else
-    int col[kExpandCap];
 #endif
+    };
     alignas(8) unsigned char row[kExpandCap];  // tile-local row of batch value q
     int rs[kMaxRows / R + 1];
     int mark[64];  // block-row marks of one batch
@@ -232,7 +231,11 @@ __device__ __forceinline__ void reduce_slots(ACC* acc, const unsigned char* row,
         for (int k = 0; k < K; ++k) rw[k] = k < nv ? (int)row[lane * K + k] : kNone;
     }
     // Slots past the end form a suffix, so a run is only ever closed on a
-    // real row; the last run is dropped when it is past the end.
+    // real row.  Runs that end inside the lane go to LDS at once; the lane's
+    // last run is first summed with the last runs of neighbouring lanes on
+    // the same row (DPP segmented sum inside 16-lane rows), so that a row
+    // spread over many lanes costs one LDS add, not one per lane.  Runs past
+    // the end (row kNone) are dropped.
     ACC s = p[0];
     int rp = rw[0];
 #pragma unroll
@@ -242,22 +245,8 @@ __device__ __forceinline__ void reduce_slots(ACC* acc, const unsigned char* row,
         s = change ? p[k] : s + p[k];
         rp = rw[k];
     }
-    if (rp != kNone) atomicAdd(&acc[rp], s);
-}
-
-template <class X, class ACC, int K>
-__device__ __forceinline__ void gather_x(const int* col, const unsigned char* row, const X* __restrict__ x,
-                                         int total, int lane, ACC* xv, int* rw) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int q = lane + 64 * k;
-        xv[k] = (ACC)0;
-        rw[k] = -1;
-        if (q < total) {
-            rw[k] = row[q];
-            xv[k] = (ACC)x[col[q]];
-        }
-    }
+    const bool tail = row_segmented_sum(s, rp);
+    if (tail && rp != kNone) atomicAdd(&acc[rp], s);
 }
 
 template <class ACC, int K>
@@ -383,7 +372,7 @@ template <class V, class X, class ACC, int R, int VS, int MB>
 __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __restrict__ bptr,
                                           const unsigned char* __restrict__ hdr, const V* __restrict__ val,
                                           const X* __restrict__ x, X* __restrict__ y, int br0, int br1, int vstart,
-                                          int vend, int64_t nrows, const Epilogue& ep) {
+                                          int vend, int64_t nrows, int64_t ncols, const Epilogue& ep) {
     constexpr int K = kExpandCap / 64;
     constexpr int NW = MaskWords<R * MB>::kWords;
     const int lane = lane_id();
@@ -415,22 +404,24 @@ __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __res
         load_header_raw<R, MB>(hdr, bn + lane, b1 - 1, h);  // past the end: block b1-1, masked
         const int br = batch_blockrows(sm.rs, sm.mark, nbr, top, b - b0, cur.bs.ntake, prev, lane);
         prev = __builtin_amdgcn_readlane(br, cur.bs.ntake - 1);
-        if (lane < cur.bs.ntake) expand_rows<R, VS, NW>(sm.col, sm.row, br * R, cur.cs, cur.mw, cur.bs.incl - cur.bs.cnt);
+        if (lane < cur.bs.ntake)
+            expand_rows<R, VS, NW>(sm.col, sm.row, br * R, cur.cs, cur.mw, cur.bs.incl - cur.bs.cnt);
         wave_sync();
-#if SPC5_REDUCE
         if (cur.bs.total > 0) {
             ACC xv[K];
             request_x<X, ACC, K>(sm, x, cur.bs.total, lane, xv);
+#if SPC5_REDUCE
             reduce_batch<ACC, K>(sm, a, xv, cur.bs.total, lane);
-        }
 #else
-        {
-            ACC xv[K];
             int rw[K];
-            gather_x<X, ACC, K>(sm.col, sm.row, x, cur.bs.total, lane, xv, rw);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int q = lane + 64 * k;
+                rw[k] = q < cur.bs.total ? (int)sm.row[q] : -1;
+            }
             accumulate_dpp<ACC, K>(sm.acc, cur.bs.total, a, xv, rw);
-        }
 #endif
+        }
         v += cur.bs.total;
         b = bn;
         wave_sync();
