@@ -162,47 +162,68 @@ def solve_benchmark(n=160, r=4, rtol=1e-10):
     out["fp64_cg"] = {"seconds": round(t64, 4), "iterations": st["iterations"], "relres": st["relres"]}
     log(f"[bench] fp64 CG: {t64 * 1e3:.1f} ms, {st['iterations']} it, relres {st['relres']:.2e}")
     nx64 = float(torch.linalg.vector_norm(x64))
+    # block formats the low-precision copy may use: the double matrix's own r
+    # and the others, converted from the same CSR; the fastest product wins
+    d_csr = [torch.from_numpy(a).cuda() for a in (rp, col, val)]
+    shapes = {r: A64}
+    for rr in (1, 2, 4):
+        if rr not in shapes:
+            shapes[rr] = DeviceMatrix.from_csr(*d_csr, rr)
+    bf = b.to(torch.float32)
+    y64 = torch.zeros(N, dtype=torch.float64, device="cuda")
+    ms_64, _ = time_spmv(A64, bf.to(torch.float64), y64, 20, 3)
     for kind in ("f32", "bf16"):
-        Alo = lowp(A64, kind)
+        ylo = torch.zeros(N, dtype=torch.float32, device="cuda")
+        cands = {rr: lowp(m, kind) for rr, m in shapes.items()}
+        times = {rr: time_spmv(m, bf, ylo, 20, 3)[0] for rr, m in cands.items()}
+        r_lo = min(times, key=times.get)
+        Alo = cands[r_lo]
+        ms_lo = times[r_lo]
+        del cands
+        # one product in reduced precision against the double product
+        ylo.zero_()
+        Alo.spmv(bf, ylo)
+        y64.zero_()
+        A64.spmv(bf.to(torch.float64), y64)
+        spmv_err = float((ylo.double() - y64).abs().max() / (54.0 * float(bf.abs().max())))
+        log(f"[bench] {kind} SpMV beta({r_lo},8) {ms_lo:.4f} ms vs fp64 beta({r},8) {ms_64:.4f} ms "
+            f"(candidates {', '.join(f'r={k}: {v:.4f}' for k, v in sorted(times.items()))})")
+        out[f"{kind}_matrix"] = {"r": r_lo, "value_bytes": Alo.values.element_size(),
+                                 "bytes_per_spmv": Alo.algorithmic_bytes(), "spmv_ms": round(ms_lo, 4),
+                                 "fp64_spmv_ms": round(ms_64, 4),
+                                 "spmv_ms_by_r": {str(k): round(v, 4) for k, v in times.items()},
+                                 "spmv_normwise_err_vs_fp64": spmv_err}
         ir_cg(A64, Alo, b, rtol=1e-2, max_outer=1, max_inner=16)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         x, st = ir_cg(A64, Alo, b, rtol=rtol)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
-        # one product in reduced precision against the double product
-        bf = b.to(torch.float32)
-        ylo = torch.zeros(N, dtype=torch.float32, device="cuda")
-        Alo.spmv(bf, ylo)
-        y64 = torch.zeros(N, dtype=torch.float64, device="cuda")
-        A64.spmv(bf.to(torch.float64), y64)
-        spmv_err = float((ylo.double() - y64).abs().max() / (54.0 * float(bf.abs().max())))
-        ms_lo, _ = time_spmv(Alo, bf, ylo, 20, 3)
-        ms_64, _ = time_spmv(A64, bf.to(torch.float64), y64, 20, 3)
         out[f"ir_cg_{kind}"] = {
             "seconds": round(t, 4), "speedup_vs_fp64": round(t64 / t, 3), "inner_iterations": st["iterations"],
             "outer": st["outer"], "relres": st["relres"],
-            "x_rel_diff_vs_fp64_cg": float(torch.linalg.vector_norm(x - x64)) / nx64,
-            "spmv_normwise_err_vs_fp64": spmv_err,
-            "value_bytes": Alo.values.element_size(), "bytes_per_spmv": Alo.algorithmic_bytes(),
-            "spmv_ms": round(ms_lo, 4), "fp64_spmv_ms": round(ms_64, 4)}
-        log(f"[bench] {kind} SpMV {ms_lo:.4f} ms vs fp64 {ms_64:.4f} ms")
+            "x_rel_diff_vs_fp64_cg": float(torch.linalg.vector_norm(x - x64)) / nx64}
         log(f"[bench] IR-CG {kind}: {t * 1e3:.1f} ms ({t64 / t:.2f}x), {st['outer']} outer / "
             f"{st['iterations']} inner, relres {st['relres']:.2e}")
-        # reliable-update CG on the same pair of matrices
-        mixed_cg(A64, Alo, b, rtol=1e-2, maxit=16)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        x, st = mixed_cg(A64, Alo, b, rtol=rtol)
-        torch.cuda.synchronize()
-        t = time.perf_counter() - t0
-        out[f"mixed_cg_{kind}"] = {
-            "seconds": round(t, 4), "speedup_vs_fp64": round(t64 / t, 3), "iterations": st["iterations"],
-            "reliable_updates": st["outer"], "relres": st["relres"],
-            "x_rel_diff_vs_fp64_cg": float(torch.linalg.vector_norm(x - x64)) / nx64}
-        log(f"[bench] mixed CG {kind}: {t * 1e3:.1f} ms ({t64 / t:.2f}x), {st['iterations']} it / "
-            f"{st['outer']} reliable updates, relres {st['relres']:.2e}")
+        # reliable-update CG on the same pair of matrices: float vectors with a
+        # double tail (the default), without the tail, and with double vectors
+        for tag, kw in (("", {}), ("_no_tail", {"tail_rtol": 0.0}), ("_vec64", {"vectors": "f64"})):
+            mixed_cg(A64, Alo, b, rtol=1e-2, maxit=16, **kw)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x, st = mixed_cg(A64, Alo, b, rtol=rtol, **kw)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            out[f"mixed_cg_{kind}{tag}"] = {
+                "seconds": round(t, 4), "speedup_vs_fp64": round(t64 / t, 3), "iterations": st["iterations"],
+                "double_tail_iterations": st["tail_iterations"], "reliable_updates": st["outer"],
+                "relres": st["relres"], "x_rel_diff_vs_fp64_cg": float(torch.linalg.vector_norm(x - x64)) / nx64,
+                "options": kw}
+            log(f"[bench] mixed CG {kind}{tag}: {t * 1e3:.1f} ms ({t64 / t:.2f}x), {st['iterations']} it "
+                f"({st['tail_iterations']} in the double tail) / {st['outer']} reliable updates, "
+                f"relres {st['relres']:.2e}")
         del Alo
+    del shapes, d_csr
     out["fp64_bytes_per_spmv"] = A64.algorithmic_bytes()
     del A64
     torch.cuda.empty_cache()

@@ -163,10 +163,11 @@ int spc5_hip_spmv_ex(spc5_hip_mat_t h, double alpha, const void* d_x, double bet
 /* ---------------------------------------------------------------------- */
 
 typedef struct {
-    int iterations;           /* CG iterations (for IR: float CG iterations in total) */
-    int outer;                /* refinement steps (IR only) */
-    double relres;            /* ||b - A x|| / ||b||, recomputed at the end */
-    double recurrence_relres; /* reserved, 0 */
+    int iterations;      /* CG iterations (IR: float CG iterations in total; mixed CG: all iterations) */
+    int outer;           /* refinement steps (IR) or reliable updates (mixed CG) */
+    double relres;       /* ||b - A x|| / ||b||, recomputed at the end */
+    int tail_iterations; /* mixed CG: iterations of the double tail (included in `iterations`) */
+    int reserved;        /* 0 */
 } spc5_hip_solve_stats;
 
 /* d_out[i] = d_in[i] rounded to float (to_type F32) or to bfloat16 through
@@ -189,16 +190,20 @@ int spc5_hip_cg(spc5_hip_mat_t A, const void* d_b, void* d_x, double rtol, int m
 int spc5_hip_ir_cg(spc5_hip_mat_t A64, spc5_hip_mat_t Alo, const double* d_b, double* d_x, double rtol,
                    double inner_rtol, int max_outer, int max_inner, spc5_hip_solve_stats* st, void* stream);
 
-/* Mixed CG with reliable updates: CG runs on float vectors with A_lo (float
- * or bfloat16 values) and accumulates a float correction d.  Every 8
+/* Mixed CG with reliable updates: CG iterates with A_lo (float or bfloat16
+ * values, products and row sums in float).  vec64 = 0: the iteration's
+ * vectors are float and a float correction d is accumulated; vec64 = 1: the
+ * vectors (and x) stay double, only the matrix is float.  Every 8
  * iterations, when the iterated residual has dropped by `delta` (0 < delta
  * < 1) since the last update, d is folded into x (double), r = b - A64 x is
  * recomputed with the double SpMV and replaces the iterated residual, and
- * CG goes on with its search direction kept.  Stops at
- * ||b - A64 x|| <= rtol ||b|| or after maxit float iterations;
- * st->iterations counts float iterations, st->outer reliable updates. */
+ * CG goes on with its search direction kept.  Once the true residual is
+ * below tail_rtol ||b|| (0: never), the rest of the solve is double CG with
+ * A64, continuing from the same search direction.  Stops at
+ * ||b - A64 x|| <= rtol ||b|| or after maxit iterations in total. */
 int spc5_hip_mixed_cg(spc5_hip_mat_t A64, spc5_hip_mat_t Alo, const double* d_b, double* d_x, double rtol,
-                      double delta, int maxit, spc5_hip_solve_stats* st, void* stream);
+                      double delta, int maxit, int vec64, double tail_rtol, spc5_hip_solve_stats* st,
+                      void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Batching form: many independent matrices in one persistent launch      */

@@ -47,7 +47,7 @@ SIGNATURES = {
     "spc5_hip_values_convert": (_I, [_I, _P, _P, _I64, _P]),
     "spc5_hip_cg": (_I, [_P, _P, _P, C.c_double, _I, _P, _P]),
     "spc5_hip_ir_cg": (_I, [_P, _P, _P, _P, C.c_double, C.c_double, _I, _I, _P, _P]),
-    "spc5_hip_mixed_cg": (_I, [_P, _P, _P, _P, C.c_double, C.c_double, _I, _P, _P]),
+    "spc5_hip_mixed_cg": (_I, [_P, _P, _P, _P, C.c_double, C.c_double, _I, _I, C.c_double, _P, _P]),
     "spc5_hip_spmm": (_I, [_P, _I, _I, C.c_double, _P, _I64, C.c_double, _P, _I64, _P]),
     "spc5_hip_synth_banded_count": (_I, [_I64, _I64, C.c_uint64, C.c_uint64, _P, C.POINTER(_I64), _P]),
     "spc5_hip_synth_banded_fill": (_I, [_I64, _I64, C.c_uint64, C.c_uint64, _P, _P, _P, _P]),
@@ -62,7 +62,7 @@ SIGNATURES = {
 class SolveStats(C.Structure):
     """spc5_hip_solve_stats"""
     _fields_ = [("iterations", C.c_int), ("outer", C.c_int), ("relres", C.c_double),
-                ("recurrence_relres", C.c_double)]
+                ("tail_iterations", C.c_int), ("reserved", C.c_int)]
 
 _lib = None
 

@@ -81,6 +81,10 @@ namespace spc5 {
 
 // Launch y = alpha A x + beta y for a handle (defined in spmv.hip).
 int launch_spmv(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s, const Epilogue& ep = Epilogue());
+// The same product with double x and y whatever the value type (float and
+// bfloat16 values are multiplied and summed in float; defined in spmv.hip).
+int launch_spmv_f64vec(const spc5_hip_mat& m, const double* x, double* y, hipStream_t s,
+                       const Epilogue& ep = Epilogue());
 // Build the tile plan of a handle (defined in spmv.hip).
 int build_plan(spc5_hip_mat& m, hipStream_t s);
 // Another tile list of the same matrix with at most `maxrows` rows per tile

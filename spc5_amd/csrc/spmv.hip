@@ -205,6 +205,16 @@ int launch_spmv(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s, co
     return SPC5_HIP_ERR_INVALID;
 }
 
+int launch_spmv_f64vec(const spc5_hip_mat& m, const double* x, double* y, hipStream_t s, const Epilogue& ep) {
+    if (m.plan.ntiles == 0) return SPC5_HIP_OK;
+    if (m.value_type == SPC5_HIP_F64) return launch_spmv_vs<double, double, double, 8, 1>(m, x, y, s, ep);
+    if (m.value_type == SPC5_HIP_F32 && m.vs == 16) return launch_spmv_vs<float, double, FloatAcc, 16, 2>(m, x, y, s, ep);
+    if (m.value_type == SPC5_HIP_F32 && m.vs == 8) return launch_spmv_vs<float, double, FloatAcc, 8, 1>(m, x, y, s, ep);
+    if (m.value_type == SPC5_HIP_BF16) return launch_spmv_vs<unsigned short, double, FloatAcc, 8, 1>(m, x, y, s, ep);
+    set_error("unsupported value type / VS");
+    return SPC5_HIP_ERR_INVALID;
+}
+
 int build_plan(spc5_hip_mat& m, hipStream_t s) {
     Plan& p = m.plan;
     p.ntiles = 0;

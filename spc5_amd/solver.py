@@ -68,17 +68,26 @@ def ir_cg(A64: DeviceMatrix, Alo: DeviceMatrix, b, x=None, rtol=1e-10, inner_rto
     return x, {"iterations": st.iterations, "outer": st.outer, "relres": st.relres}
 
 
-def mixed_cg(A64: DeviceMatrix, Alo: DeviceMatrix, b, x=None, rtol=1e-10, delta=None, maxit=20000, stream=None):
-    """CG with reliable updates: float iteration on Alo, double solution and
-    residual (b - A64 x recomputed whenever the float residual has dropped by
-    `delta`, default 0.1 for float values and 0.5 for bfloat16)."""
+def mixed_cg(A64: DeviceMatrix, Alo: DeviceMatrix, b, x=None, rtol=1e-10, delta=0.1, maxit=20000, vectors="f32",
+             tail_rtol=None, stream=None):
+    """CG with reliable updates: the iteration multiplies by Alo (float or
+    bfloat16 values, float products and sums); its vectors are float
+    (vectors="f32", with a float correction folded into x at every update)
+    or double ("f64").  b - A64 x is recomputed in double whenever the
+    iterated residual has dropped by `delta`.  Below tail_rtol (default 1e-6
+    for float values, 1e-3 for bfloat16; 0 = never) the solve finishes as
+    double CG with A64 from the same search direction."""
     _need_cuda(b, "b", torch.float64)
     if x is None:
         x = torch.zeros_like(b)
     _need_cuda(x, "x", torch.float64)
-    if delta is None:
-        delta = 0.5 if Alo.values.dtype == torch.bfloat16 else 0.1
+    if vectors not in ("f64", "f32"):
+        raise Spc5Error("vectors must be 'f64' or 'f32'")
+    if tail_rtol is None:
+        tail_rtol = 1e-3 if Alo.values.dtype == torch.bfloat16 else 1e-6
     st = SolveStats()
     check(lib().spc5_hip_mixed_cg(A64._h, Alo._h, _ptr(b), _ptr(x), float(rtol), float(delta), int(maxit),
-                                  C.byref(st), stream_ptr(stream)), "spc5_hip_mixed_cg")
-    return x, {"iterations": st.iterations, "outer": st.outer, "relres": st.relres}
+                                  1 if vectors == "f64" else 0, float(tail_rtol), C.byref(st), stream_ptr(stream)),
+          "spc5_hip_mixed_cg")
+    return x, {"iterations": st.iterations, "outer": st.outer, "relres": st.relres,
+               "tail_iterations": st.tail_iterations}

@@ -124,9 +124,11 @@ def test_ir_cg_mixed(kind, r):
     assert np.linalg.norm(xh - xs) / np.linalg.norm(xs) <= 1e-7
 
 
+@pytest.mark.parametrize("tail", [None, 0.0])
+@pytest.mark.parametrize("vectors", ["f64", "f32"])
 @pytest.mark.parametrize("kind", ["f32", "bf16"])
 @pytest.mark.parametrize("r", (1, 4))
-def test_mixed_cg_reliable_updates(kind, r):
+def test_mixed_cg_reliable_updates(kind, r, vectors, tail):
     """Reliable-update CG: float iteration, double residual/solution, the
     true residual below 1e-10 and the solution equal to a dense solve."""
     from spc5_amd.device import DeviceMatrix
@@ -136,8 +138,12 @@ def test_mixed_cg_reliable_updates(kind, r):
     A64 = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val), r)
     Alo = lowp(A64, kind)
     b = np.random.default_rng(6).uniform(-1, 1, n)
-    x, st = mixed_cg(A64, Alo, dev(b), rtol=1e-10)
+    x, st = mixed_cg(A64, Alo, dev(b), rtol=1e-10, vectors=vectors, tail_rtol=tail)
     xh = x.cpu().numpy()
+    if tail == 0.0:
+        assert st["tail_iterations"] == 0
+    else:
+        assert 0 < st["tail_iterations"] < st["iterations"]
     Ad = csr_dense(rp, col, val, n)
     relres = np.linalg.norm(b - Ad @ xh) / np.linalg.norm(b)
     assert relres <= 1e-10, (relres, st)
@@ -145,6 +151,33 @@ def test_mixed_cg_reliable_updates(kind, r):
     assert st["outer"] >= 2  # at least one intermediate update besides the last
     xs = np.linalg.solve(Ad, b)
     assert np.linalg.norm(xh - xs) / np.linalg.norm(xs) <= 1e-7
+
+
+def test_cg_graph_replay_matches_direct_launches(monkeypatch):
+    """The CG chunk recorded into a hipGraph and replayed gives the same
+    iterates, bit for bit, as launching every kernel directly (double CG and
+    the reliable-update solver), over many replays."""
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import cg, lowp, mixed_cg
+    rp, col, val = stencil27_small(12)
+    n = len(rp) - 1
+    A64 = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val), 4)
+    Alo = lowp(A64, "f32")
+    b = dev(np.random.default_rng(11).uniform(-1, 1, n))
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("SPC5_CG_GRAPH", flag)
+        x1, s1 = cg(A64, b, rtol=1e-12, maxit=500)
+        x2, s2 = mixed_cg(A64, Alo, b, rtol=1e-11, vectors="f32")
+        x3, s3 = mixed_cg(A64, Alo, b, rtol=1e-11, vectors="f64")
+        torch.cuda.synchronize()
+        out[flag] = (x1.cpu().numpy(), s1, x2.cpu().numpy(), s2, x3.cpu().numpy(), s3)
+    g, d = out["1"], out["0"]
+    assert g[1]["iterations"] == d[1]["iterations"] >= 16
+    assert np.array_equal(g[0], d[0])
+    for i in (2, 4):
+        assert g[i + 1]["iterations"] == d[i + 1]["iterations"] and g[i + 1]["outer"] == d[i + 1]["outer"]
+        assert np.array_equal(g[i], d[i])
 
 
 def test_mixed_cg_nonzero_guess_and_limits():
@@ -167,6 +200,8 @@ def test_mixed_cg_nonzero_guess_and_limits():
     assert np.linalg.norm(b - Ad @ x.cpu().numpy()) / np.linalg.norm(b) <= 1e-10
     with pytest.raises(Spc5Error):
         mixed_cg(A64, Alo, dev(b), rtol=1e-10, delta=1.5)
+    with pytest.raises(Spc5Error):
+        mixed_cg(A64, Alo, dev(b), rtol=1e-10, vectors="f16")
     with pytest.raises(Spc5Error):
         mixed_cg(A64, A64, dev(b), rtol=1e-10)
 
