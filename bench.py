@@ -133,12 +133,14 @@ def cpu_baseline(rp, col, val, r, ncols, nthreads, nloops=16, max_rows=None):
             "ms_per_spmv": round(tspmv / nloops * 1e3, 3)}
 
 
-def solve_benchmark(n=160, r=4, rtol=1e-10):
+def solve_benchmark(n=160, rtol=1e-10):
     """Precision form on the 27-point stencil n^3 (diag 26.001, off -1, SPD):
     wall time to relative residual rtol of double CG and of the two mixed-
     precision solvers (restarted IR-CG and reliable-update CG) with float and
     bfloat16 values, plus errors against the double
-    solution and of one reduced-precision product against the double one."""
+    solution and of one reduced-precision product against the double one.
+    Every matrix (the double one included) uses the block shape r in
+    {1, 2, 4} and row-sort setting whose product is fastest."""
     import numpy as np
     import torch
     from spc5_amd import synth
@@ -147,11 +149,34 @@ def solve_benchmark(n=160, r=4, rtol=1e-10):
 
     rp, col, val = synth.stencil27(n)
     N = len(rp) - 1
-    A64 = DeviceMatrix.from_csr(torch.from_numpy(rp).cuda(), torch.from_numpy(col).cuda(),
-                                torch.from_numpy(val).cuda(), r)
+    d_csr = [torch.from_numpy(a).cuda() for a in (rp, col, val)]
+    shapes = {rr: DeviceMatrix.from_csr(*d_csr, rr) for rr in (1, 2, 4)}
     b = torch.from_numpy(synth.uniform(N, 7)).cuda()
-    out = {"workload": f"stencil27 {n}^3 (N={N}, nnz={len(val)}), diag 26.001, beta({r},8), b U[-1,1] seed 7",
-           "rtol": rtol}
+    bf = b.to(torch.float32)
+
+    def fastest(cands, x, y):
+        """(r, row_sort) -> matrix; returns the key of the fastest product and all times."""
+        times = {}
+        for (rr, srt), m in cands.items():
+            m.row_sort = srt
+            times[(rr, srt)] = time_spmv(m, x, y, 20, 3)[0]
+        return min(times, key=times.get), times
+
+    def keyname(k):
+        return f"r={k[0]}{' sorted' if k[1] else ''}"
+
+    y64 = torch.zeros(N, dtype=torch.float64, device="cuda")
+    k64, times64 = fastest({(rr, srt): m for rr, m in shapes.items() for srt in ((False, True) if rr > 1 else (False,))},
+                           bf.to(torch.float64), y64)
+    A64 = shapes[k64[0]]
+    A64.row_sort = k64[1]
+    r = k64[0]
+    ms_64 = times64[k64]
+    out = {"workload": f"stencil27 {n}^3 (N={N}, nnz={len(val)}), diag 26.001, b U[-1,1] seed 7", "rtol": rtol,
+           "fp64_matrix": {"r": r, "row_sort": k64[1], "spmv_ms": round(ms_64, 4),
+                           "spmv_ms_by_shape": {keyname(k): round(v, 4) for k, v in times64.items()}}}
+    log(f"[bench] fp64 SpMV {keyname(k64)} {ms_64:.4f} ms "
+        f"(candidates {', '.join(f'{keyname(k)}: {v:.4f}' for k, v in sorted(times64.items()))})")
     # warm-up of every code path on a short solve
     cg(A64, b, rtol=1e-2, maxit=16)
     torch.cuda.synchronize()
@@ -162,23 +187,16 @@ def solve_benchmark(n=160, r=4, rtol=1e-10):
     out["fp64_cg"] = {"seconds": round(t64, 4), "iterations": st["iterations"], "relres": st["relres"]}
     log(f"[bench] fp64 CG: {t64 * 1e3:.1f} ms, {st['iterations']} it, relres {st['relres']:.2e}")
     nx64 = float(torch.linalg.vector_norm(x64))
-    # block formats the low-precision copy may use: the double matrix's own r
-    # and the others, converted from the same CSR; the fastest product wins
-    d_csr = [torch.from_numpy(a).cuda() for a in (rp, col, val)]
-    shapes = {r: A64}
-    for rr in (1, 2, 4):
-        if rr not in shapes:
-            shapes[rr] = DeviceMatrix.from_csr(*d_csr, rr)
-    bf = b.to(torch.float32)
-    y64 = torch.zeros(N, dtype=torch.float64, device="cuda")
-    ms_64, _ = time_spmv(A64, bf.to(torch.float64), y64, 20, 3)
+    # block formats the low-precision copy may use: the same shapes, values
+    # converted; the fastest product wins
     for kind in ("f32", "bf16"):
         ylo = torch.zeros(N, dtype=torch.float32, device="cuda")
-        cands = {rr: lowp(m, kind) for rr, m in shapes.items()}
-        times = {rr: time_spmv(m, bf, ylo, 20, 3)[0] for rr, m in cands.items()}
-        r_lo = min(times, key=times.get)
-        Alo = cands[r_lo]
-        ms_lo = times[r_lo]
+        cands = {(rr, srt): lowp(m, kind) for rr, m in shapes.items()
+                 for srt in ((False, True) if rr > 1 else (False,))}
+        klo, times = fastest(cands, bf, ylo)
+        r_lo = klo[0]
+        Alo = cands[klo]
+        ms_lo = times[klo]
         del cands
         # one product in reduced precision against the double product
         ylo.zero_()
@@ -186,12 +204,12 @@ def solve_benchmark(n=160, r=4, rtol=1e-10):
         y64.zero_()
         A64.spmv(bf.to(torch.float64), y64)
         spmv_err = float((ylo.double() - y64).abs().max() / (54.0 * float(bf.abs().max())))
-        log(f"[bench] {kind} SpMV beta({r_lo},8) {ms_lo:.4f} ms vs fp64 beta({r},8) {ms_64:.4f} ms "
-            f"(candidates {', '.join(f'r={k}: {v:.4f}' for k, v in sorted(times.items()))})")
-        out[f"{kind}_matrix"] = {"r": r_lo, "value_bytes": Alo.values.element_size(),
+        log(f"[bench] {kind} SpMV beta({r_lo},8) {keyname(klo)} {ms_lo:.4f} ms vs fp64 {keyname(k64)} "
+            f"{ms_64:.4f} ms (candidates {', '.join(f'{keyname(k)}: {v:.4f}' for k, v in sorted(times.items()))})")
+        out[f"{kind}_matrix"] = {"r": r_lo, "row_sort": klo[1], "value_bytes": Alo.values.element_size(),
                                  "bytes_per_spmv": Alo.algorithmic_bytes(), "spmv_ms": round(ms_lo, 4),
                                  "fp64_spmv_ms": round(ms_64, 4),
-                                 "spmv_ms_by_r": {str(k): round(v, 4) for k, v in times.items()},
+                                 "spmv_ms_by_shape": {keyname(k): round(v, 4) for k, v in times.items()},
                                  "spmv_normwise_err_vs_fp64": spmv_err}
         ir_cg(A64, Alo, b, rtol=1e-2, max_outer=1, max_inner=16)
         torch.cuda.synchronize()
@@ -205,9 +223,11 @@ def solve_benchmark(n=160, r=4, rtol=1e-10):
             "x_rel_diff_vs_fp64_cg": float(torch.linalg.vector_norm(x - x64)) / nx64}
         log(f"[bench] IR-CG {kind}: {t * 1e3:.1f} ms ({t64 / t:.2f}x), {st['outer']} outer / "
             f"{st['iterations']} inner, relres {st['relres']:.2e}")
-        # reliable-update CG on the same pair of matrices: float vectors with a
-        # double tail (the default), without the tail, and with double vectors
-        for tag, kw in (("", {}), ("_no_tail", {"tail_rtol": 0.0}), ("_vec64", {"vectors": "f64"})):
+        # reliable-update CG on the same pair of matrices: the defaults (float
+        # vectors; a double tail for bfloat16 only), with and without a tail,
+        # and with double vectors
+        tail_alt = ("_tail", {"tail_rtol": 1e-6}) if kind == "f32" else ("_no_tail", {"tail_rtol": 0.0})
+        for tag, kw in (("", {}), tail_alt, ("_vec64", {"vectors": "f64"})):
             mixed_cg(A64, Alo, b, rtol=1e-2, maxit=16, **kw)
             torch.cuda.synchronize()
             t0 = time.perf_counter()

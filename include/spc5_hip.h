@@ -128,6 +128,16 @@ int spc5_hip_mat_destroy(spc5_hip_mat_t h);
 int spc5_hip_mat_info(spc5_hip_mat_t h, int64_t* ntiles, int64_t* nblockrows, int64_t* nblocks,
                       int64_t* nnz);
 
+/* Per-handle kernel options (spc5_hip_mat_set_option).
+ * ROW_SORT (0 or 1, default 0; r > 1 only): the products of every batch of
+ * blocks are summed in row order instead of block order.  Pays off where
+ * every block of a block-row covers the same rows (stencils: all blocks of
+ * a block-row add to the same r rows at the same step), costs extra work
+ * for irregular masks.  Results agree with the default order to rounding.
+ * No counterpart in the reference (a property of the GPU kernel). */
+#define SPC5_HIP_OPT_ROW_SORT 1
+int spc5_hip_mat_set_option(spc5_hip_mat_t h, int option, int64_t value);
+
 /* y += A x on device pointers (x and y in the matrix's vector type).
  * Replaces core_SPC5_<r>rVc_Spmv_<T> (src/spc5.hpp:2309-2715) and the
  * OpenMP driver SPC5_<r>rVc_Spmv_omp (src/spc5.hpp:557-599 etc.). */

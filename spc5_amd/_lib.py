@@ -16,6 +16,7 @@ SPC5_HIP_OK = 0
 SPC5_HIP_F64 = 0
 SPC5_HIP_F32 = 1
 SPC5_HIP_BF16 = 2
+SPC5_HIP_OPT_ROW_SORT = 1
 
 _P = C.c_void_p
 _I = C.c_int
@@ -36,6 +37,7 @@ SIGNATURES = {
     "spc5_hip_mat_create_values_view": (_I, [_P, _I, _P, C.POINTER(_P)]),
     "spc5_hip_mat_destroy": (_I, [_P]),
     "spc5_hip_mat_info": (_I, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64)]),
+    "spc5_hip_mat_set_option": (_I, [_P, _I, _I64]),
     "spc5_hip_spmv": (_I, [_P, _P, _P, _P]),
     "spc5_hip_convert_count": (_I, [_I, _I, _I64, _P, _P, _P, C.POINTER(_I64), _P]),
     "spc5_hip_convert_fill": (_I, [_I, _I, _I, _I, _I64, _P, _P, _P, _P, _P, _P, _P]),

@@ -95,6 +95,16 @@ __device__ __forceinline__ int wave_incl_max(int v) {
     return max(v, row == 0 ? 0 : (row == 1 ? c1 : (row == 2 ? c2 : c3)));
 }
 
+// v of lane-1 across the whole wave (GFX9 wave_shr:1); `first` on lane 0.
+__device__ __forceinline__ int wave_prev_lane(int v, int first) {
+    return __builtin_amdgcn_update_dpp(first, v, 0x138, 0xf, 0xf, false);
+}
+
+// Number of set bits of m below this lane.
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+}
+
 // DPP moves of 32/64-bit values inside 16-lane rows (0 where the source
 // lane is outside the row).
 constexpr int kDppRowShl1 = 0x101;
@@ -228,6 +238,12 @@ __device__ __forceinline__ V load_stream(const V* p) {
     return *p;
 #endif
 }
+
+// Cache-policy operand of the raw buffer loads of the value stream (the nt
+// bit on gfx94x/gfx950), and word 3 of a raw (stride 0) buffer resource on
+// gfx9-family GPUs.
+constexpr int kNonTemporal = SPC5_NT_VALUES ? 2 : 0;
+constexpr int kRawBufferFlags = 0x00020000;
 template <class V, class ACC>
 struct ValueLoad {
     __device__ __forceinline__ static ACC get(const V* p) { return (ACC)load_stream(p); }

@@ -73,6 +73,7 @@ struct spc5_hip_mat {
     const int* bptr = nullptr;            // device, [nbr+1]
     const unsigned char* hdr = nullptr;   // device, [nblocks*(4+r*mb)]
     const void* values = nullptr;         // device, [nnz]
+    int row_sort = 0;                     // SPC5_HIP_OPT_ROW_SORT
     spc5::Plan plan;
     std::vector<std::shared_ptr<spc5::DeviceBuffer>> owned;  // buffers the handle allocated itself
 };

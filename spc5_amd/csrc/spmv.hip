@@ -49,19 +49,19 @@ __device__ __forceinline__ int64_t xcd_tile(int wave) {
 #define SPC5_WAVES_PER_EU 1
 #endif
 
-template <class V, class X, class ACC, int R, int VS, int MB>
+template <class V, class X, class ACC, int R, int VS, int MB, bool SORT>
 __global__ __launch_bounds__(kWavesPerGroup * 64, SPC5_WAVES_PER_EU) void k_spmv_tiles(
     const int* __restrict__ bptr, const unsigned char* __restrict__ hdr, const V* __restrict__ val,
     const X* __restrict__ x, X* __restrict__ y, const int* __restrict__ tile_br,
     const int* __restrict__ tile_val, int64_t ntiles, int64_t nrows, int64_t ncols, Epilogue ep) {
-    __shared__ TileSmem<ACC, R> sm[kWavesPerGroup];
+    __shared__ TileSmem<ACC, R, SORT> sm[kWavesPerGroup];
     const int w = threadIdx.x >> 6;
     const int64_t t = xcd_tile(w);
     if (t >= ntiles) return;  // whole wave leaves; no workgroup barrier is used
     Epilogue e = ep;
     if (e.dot) e.dot += t;  // one partial per tile
-    spmv_tile<V, X, ACC, R, VS, MB>(sm[w], bptr, hdr, val, x, y, tile_br[t], tile_br[t + 1], tile_val[t],
-                                    tile_val[t + 1], nrows, ncols, e);
+    spmv_tile<V, X, ACC, R, VS, MB, SORT>(sm[w], bptr, hdr, val, x, y, tile_br[t], tile_br[t + 1], tile_val[t],
+                                          tile_val[t + 1], nrows, ncols, e);
 }
 
 // ---- plan kernels --------------------------------------------------------
@@ -142,32 +142,32 @@ void launch_blockrow_values(const spc5_hip_mat& m, int* cnt, int* err, hipStream
                        err);
 }
 
-template <class V, class X, class ACC, int VS, int MB>
-int launch_spmv_vs(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s, const Epilogue& ep) {
+template <class V, class X, class ACC, int R, int VS, int MB, bool SORT>
+void launch_tiles(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s, const Epilogue& ep) {
     const int64_t groups = (m.plan.ntiles + kWavesPerGroup - 1) / kWavesPerGroup;
     const unsigned grid = (unsigned)((groups + 7) / 8 * 8);
-    const dim3 blk(kWavesPerGroup * 64);
-    const int* tb = m.plan.tile_br->as<int>();
-    const int* tv = m.plan.tile_val->as<int>();
-    const V* val = reinterpret_cast<const V*>(m.values);
-    const X* xx = reinterpret_cast<const X*>(x);
-    X* yy = reinterpret_cast<X*>(y);
+    hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, R, VS, MB, SORT>), dim3(grid), dim3(kWavesPerGroup * 64), 0, s,
+                       m.bptr, m.hdr, reinterpret_cast<const V*>(m.values), reinterpret_cast<const X*>(x),
+                       reinterpret_cast<X*>(y), m.plan.tile_br->as<int>(), m.plan.tile_val->as<int>(),
+                       m.plan.ntiles, m.nrows, m.ncols, ep);
+}
+
+template <class V, class X, class ACC, int VS, int MB>
+int launch_spmv_vs(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s, const Epilogue& ep) {
+    const bool sort = m.row_sort != 0;
     switch (m.r) {
-    case 1:
-        hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 1, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
-                           tb, tv, m.plan.ntiles, m.nrows, m.ncols, ep);
-        break;
+    case 1: launch_tiles<V, X, ACC, 1, VS, MB, false>(m, x, y, s, ep); break;
     case 2:
-        hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 2, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
-                           tb, tv, m.plan.ntiles, m.nrows, m.ncols, ep);
+        if (sort) launch_tiles<V, X, ACC, 2, VS, MB, true>(m, x, y, s, ep);
+        else launch_tiles<V, X, ACC, 2, VS, MB, false>(m, x, y, s, ep);
         break;
     case 4:
-        hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 4, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
-                           tb, tv, m.plan.ntiles, m.nrows, m.ncols, ep);
+        if (sort) launch_tiles<V, X, ACC, 4, VS, MB, true>(m, x, y, s, ep);
+        else launch_tiles<V, X, ACC, 4, VS, MB, false>(m, x, y, s, ep);
         break;
     case 8:
-        hipLaunchKernelGGL((k_spmv_tiles<V, X, ACC, 8, VS, MB>), dim3(grid), blk, 0, s, m.bptr, m.hdr, val, xx, yy,
-                           tb, tv, m.plan.ntiles, m.nrows, m.ncols, ep);
+        if (sort) launch_tiles<V, X, ACC, 8, VS, MB, true>(m, x, y, s, ep);
+        else launch_tiles<V, X, ACC, 8, VS, MB, false>(m, x, y, s, ep);
         break;
     default:
         set_error("invalid r");
@@ -376,6 +376,19 @@ int spc5_hip_mat_info(spc5_hip_mat_t h, int64_t* ntiles, int64_t* nblockrows, in
     if (nblocks) *nblocks = h->nblocks;
     if (nnz) *nnz = h->nnz;
     return SPC5_HIP_OK;
+}
+
+int spc5_hip_mat_set_option(spc5_hip_mat_t h, int option, int64_t value) {
+    SPC5_CHECK_ARG(h != nullptr, "null handle");
+    switch (option) {
+    case SPC5_HIP_OPT_ROW_SORT:
+        SPC5_CHECK_ARG(value == 0 || value == 1, "ROW_SORT must be 0 or 1");
+        h->row_sort = (int)value;
+        return SPC5_HIP_OK;
+    default:
+        set_error("unknown option");
+        return SPC5_HIP_ERR_INVALID;
+    }
 }
 
 int spc5_hip_spmv(spc5_hip_mat_t h, const void* d_x, void* d_y, void* stream) {

@@ -7,21 +7,24 @@
 //      inside the batch; blocks are taken while the batch holds at most
 //      kExpandCap values;
 //   2. the batch's values are loaded at once, kExpandCap/64 coalesced loads
-//      per lane, straight from the value stream (no zero padding);
+//      per lane, straight from the value stream (no zero padding), through
+//      a buffer resource based at the batch's first value (constant lane
+//      offsets; the range check returns 0 past the tile's values);
 //   3. the next batch's headers are requested;
 //   4. the block-row of every block of the batch comes from one pass over
 //      the tile's block-row pointers in LDS (the block-rows that start in
 //      the batch mark their first block, a wave prefix maximum spreads the
 //      marks); each taken lane then writes, for every set mask bit, the
 //      column and the tile-local row into LDS;
+//      Optionally (kSort, r > 1) the values are given row-sorted positions
+//      at the same time (row_displacements);
 //   5. in value order, every lane gathers x for its values (neighbouring
-//      lanes read neighbouring columns) and forms the products;
-//   6. row sums, two ways (SPC5_REDUCE):
-//      1 (default) -- the products go to LDS slot-major, so that lane l then
-//        owns the kSlots consecutive values l*kSlots ...; it adds them in
-//        registers and does one LDS add per run of equal rows;
-//      0 -- DPP segmented sums over runs of equal rows inside each 16-lane
-//        row, one LDS add per run.
+//      lanes read neighbouring columns; 32-bit byte offsets from x when x
+//      spans less than 4 GiB) and forms the products;
+//   6. the products go to LDS at their (row-sorted) positions; lane l then
+//      owns the K consecutive positions l*K ...: it adds them in registers,
+//      one LDS add per run of equal rows, the lane's last run first merged
+//      with the neighbouring lanes' (row_segmented_sum).
 // At the end of the tile each row is written once: y = alpha*sum + beta*y.
 #pragma once
 
@@ -29,35 +32,43 @@
 
 namespace spc5 {
 
-#ifndef SPC5_REDUCE
-#define SPC5_REDUCE 1
+// Value stream through per-batch buffer resources (1) or lane-masked
+// global loads (0); x through 32-bit byte offsets when it spans < 4 GiB (1)
+// or always through 64-bit addresses (0).
+#ifndef SPC5_VALUE_BUF
+#define SPC5_VALUE_BUF 1
+#endif
+#ifndef SPC5_X32
+#define SPC5_X32 1
 #endif
 
-
-template <class ACC>
-struct SlotLayout {
-    static constexpr int kSlots = kExpandCap / 64;  // values per lane in the row-run phase
-    // Slot-major product layout: value q = l*kSlots + k lives at k*kStride + l.
-    // The extra entries put the 64 lanes of one value-order store on
-    // distinct banks.
-    static constexpr int kStride = sizeof(ACC) == 8 ? 68 : 72;
-    __device__ __forceinline__ static int pos(int q) { return (q % kSlots) * kStride + q / kSlots; }
+// Row-sorted batches (see row_displacements): on for r > 1 when asked for.
+template <int R, bool kSort>
+struct RowSort {
+    static constexpr bool kOn = kSort && R > 1;
+    static constexpr int kWords = (R + 2) / 3;  // packed words of three 10-bit row counts
 };
 
-template <class ACC, int R>
+constexpr int kSlots = kExpandCap / 64;  // batch positions owned by one lane in the row sums
+constexpr int kNoRow = kMaxRows;         // row byte of the positions past the batch's end
+
+
+template <class ACC, int R, bool kSort = false>
 struct TileSmem {
     ACC acc[kMaxRows];  // row sums of the tile
     // Columns in value order; every column is read before the first
     // product is stored over them.
-    union {
+    // (x byte offsets instead of columns when x spans less than 4 GiB).
+    union alignas(16) {
         int col[kExpandCap];
-#if SPC5_REDUCE
-        ACC prod[SlotLayout<ACC>::kSlots * SlotLayout<ACC>::kStride];
-#endif
+        int2 cp[RowSort<R, kSort>::kOn ? kExpandCap : 1];  // row-sorted: column and row-sorted position
+        ACC prod[kExpandCap];                       // products at their (row-sorted) positions
     };
-    alignas(8) unsigned char row[kExpandCap];  // tile-local row of batch value q
+    // Tile-local row of batch position q; kNoRow past the batch's end.
+    alignas(8) unsigned char row[kExpandCap];
     int rs[kMaxRows / R + 1];
     int mark[64];  // block-row marks of one batch
+    int segs[RowSort<R, kSort>::kOn ? 2 * RowSort<R, kSort>::kWords * 64 : 1];  // row counts at segment ends
 };
 
 __device__ __forceinline__ int find_blockrow(const int* rs, int nbr, int top, int jr) {
@@ -133,8 +144,9 @@ __device__ __forceinline__ void load_values(const V* __restrict__ val, int v, in
 
 // Every set mask bit of one block -> (column, tile-local row) in LDS,
 // starting at batch position p; rbase is the tile-local first row of the
-// block's block-row.
-template <int R, int VS, int NW>
+// block's block-row.  The column written is cs + c * kScale for block
+// column c (kScale = sizeof(X) with cs a byte offset).
+template <int R, int VS, int NW, int kScale = 1>
 __device__ __forceinline__ void expand_rows(int* col, unsigned char* row, int rbase, int cs,
                                             const unsigned long long* mw, int p) {
     if constexpr (R * VS <= 16) {
@@ -143,7 +155,7 @@ __device__ __forceinline__ void expand_rows(int* col, unsigned char* row, int rb
 #pragma unroll
         for (int bit = 0; bit < R * VS; ++bit) {
             if ((m >> bit) & 1u) {
-                col[p] = cs + (bit % VS);
+                col[p] = cs + (bit % VS) * kScale;
                 row[p] = (unsigned char)(rbase + bit / VS);
                 ++p;
             }
@@ -152,7 +164,7 @@ __device__ __forceinline__ void expand_rows(int* col, unsigned char* row, int rb
         unsigned m = (unsigned)mw[0];
         while (m) {
             const int bit = __builtin_ctz(m);
-            col[p] = cs + (bit & (VS - 1));
+            col[p] = cs + (bit & (VS - 1)) * kScale;
             row[p] = (unsigned char)(rbase + bit / VS);
             ++p;
             m &= m - 1u;
@@ -163,7 +175,7 @@ __device__ __forceinline__ void expand_rows(int* col, unsigned char* row, int rb
             unsigned long long m = mw[wd];
             while (m) {
                 const int bit = __builtin_ctzll(m) + wd * 64;
-                col[p] = cs + (bit & (VS - 1));
+                col[p] = cs + (bit & (VS - 1)) * kScale;
                 row[p] = (unsigned char)(rbase + bit / VS);
                 ++p;
                 m &= m - 1ull;
@@ -176,6 +188,113 @@ template <int R, int VS, int NW>
 __device__ __forceinline__ void expand(int* col, unsigned char* row, const int* rs, int nbr, int top, int jr,
                                        int cs, const unsigned long long* mw, int p) {
     expand_rows<R, VS, NW>(col, row, find_blockrow(rs, nbr, top, jr) * R, cs, mw, p);
+}
+
+// Values of row t (0 <= t < R) of a block mask.
+template <int R, int VS, int NW>
+__device__ __forceinline__ int row_count(const unsigned long long* mw, int t) {
+    const int bit = t * VS;
+    return __popcll((mw[bit / 64] >> (bit % 64)) & ((1ull << VS) - 1ull));
+}
+
+// Row-sorted batch order (r > 1).  In value order the blocks of one
+// block-row follow each other and each block holds its rows 0..r-1 in turn,
+// so one row's values are spread over every block of the block-row and the
+// per-run LDS adds of neighbouring lanes hit the same address.  The
+// products are therefore stored in row order inside each block-row segment
+// of the batch (every value of the segment's row 0, then of row 1, ...;
+// blocks in order inside a row).  The value at value-order position p in
+// row t of the lane's block goes to position p + d[t]; this returns d for
+// the lane's block.
+//
+// Per-row counts of the taken blocks are packed three to a word (10-bit
+// fields; a batch holds at most kExpandCap < 1024 values, so no field
+// carries) and prefix-summed over the wave; the first and last lane of
+// every block-row segment leave the prefix at the segment's start and end
+// in LDS, from which every lane gets its segment's row totals.
+template <int R, int VS, int NW>
+__device__ __forceinline__ void row_displacements(int* segs, const unsigned long long* mw, int br, int ntake, int p0,
+                                                  int lane, int* d) {
+    constexpr int kW = RowSort<R, true>::kWords;
+    static_assert(kExpandCap < 1024, "10-bit row-count fields");
+    const bool taken = lane < ntake;
+    int c[R];
+    int pk[kW] = {};
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        c[t] = taken ? row_count<R, VS, NW>(mw, t) : 0;
+        pk[t / 3] += c[t] << (10 * (t % 3));
+    }
+    int incl[kW];
+#pragma unroll
+    for (int w = 0; w < kW; ++w) incl[w] = wave_incl_scan(pk[w]);
+    const bool start = taken && wave_prev_lane(br, -1) != br;
+    const unsigned long long sb = ballot64(start);
+    int seg = lanes_below(sb) + (start ? 1 : 0) - 1;  // index of the lane's segment
+    seg = seg < 0 ? 0 : seg;
+    const bool end = taken && (lane == ntake - 1 || (((sb >> lane) >> 1) & 1ull));
+    if (start) {
+#pragma unroll
+        for (int w = 0; w < kW; ++w) segs[w * 64 + seg] = incl[w] - pk[w];
+    }
+    if (end) {
+#pragma unroll
+        for (int w = 0; w < kW; ++w) segs[(kW + w) * 64 + seg] = incl[w];
+    }
+    wave_sync();
+    int at = 0, pc = p0;  // segment position reached; value-order position of row t of the block
+    int f[kW], tot[kW], own[kW];
+#pragma unroll
+    for (int w = 0; w < kW; ++w) {
+        f[w] = segs[w * 64 + seg];
+        tot[w] = segs[(kW + w) * 64 + seg] - f[w];
+        own[w] = incl[w] - pk[w] - f[w];  // values of each row in earlier blocks of the segment
+        at += (f[w] & 0x3ff) + ((f[w] >> 10) & 0x3ff) + ((f[w] >> 20) & 0x3ff);
+    }
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const int sh = 10 * (t % 3);
+        d[t] = at + ((own[t / 3] >> sh) & 0x3ff) - pc;
+        at += (tot[t / 3] >> sh) & 0x3ff;
+        pc += c[t];
+    }
+}
+
+// expand_rows for a row-sorted batch: value p (value order) gets its
+// column and row-sorted position p + d[t] at cp[p], and its row at
+// row[p + d[t]].
+template <int R, int VS, int NW, int kScale = 1>
+__device__ __forceinline__ void expand_rows_sorted(int2* cp, unsigned char* row, int rbase, int cs,
+                                                   const unsigned long long* mw, int p, const int* d) {
+    if constexpr (R * VS <= 16) {
+        const unsigned m = (unsigned)mw[0];
+#pragma unroll
+        for (int bit = 0; bit < R * VS; ++bit) {
+            if ((m >> bit) & 1u) {
+                const int s = p + d[bit / VS];
+                cp[p] = make_int2(cs + (bit % VS) * kScale, s);
+                row[s] = (unsigned char)(rbase + bit / VS);
+                ++p;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int wd = 0; wd < NW; ++wd) {
+            unsigned long long m = mw[wd];
+            while (m) {
+                const int bit = __builtin_ctzll(m) + wd * 64;
+                const int t = bit / VS;
+                int dt = d[0];
+#pragma unroll
+                for (int u = 1; u < R; ++u) dt = t == u ? d[u] : dt;
+                const int s = p + dt;
+                cp[p] = make_int2(cs + (bit & (VS - 1)) * kScale, s);
+                row[s] = (unsigned char)(rbase + t);
+                ++p;
+                m &= m - 1ull;
+            }
+        }
+    }
 }
 
 // Block-row of block `bb + lane` (tile-local block indices) for a batch of
@@ -208,34 +327,36 @@ __device__ __forceinline__ int batch_blockrows(const int* rs, int* mark, int nbr
     return br;
 }
 
-// Row sums of one batch by lane-owned runs: lane l owns batch values
-// l*K .. l*K+K-1 (products in LDS, slot-major).  Consecutive values of one
-// row are added in registers; every run of equal rows ends with one LDS add.
+// Row sums of one batch by lane-owned runs: lane l owns batch positions
+// l*K .. l*K+K-1 (products and rows in LDS in position order; rows kNoRow
+// past the batch's end).  Consecutive values of one row are added in
+// registers; every run of equal rows ends with one LDS add.
 template <class ACC, int K>
-__device__ __forceinline__ void reduce_slots(ACC* acc, const unsigned char* row, const ACC* prod, int total,
-                                             int lane) {
-    using L = SlotLayout<ACC>;
-    constexpr int kNone = kMaxRows;  // row of the slots past the batch's end
+__device__ __forceinline__ void reduce_slots(ACC* acc, const unsigned char* row, const ACC* prod, int lane) {
     ACC p[K];
+    if constexpr (K % 2 == 0) {
+        // Pairs: 8 or 16-byte reads, lane stride 3 or 6 words (no bank conflict).
+        using P2 = typename std::conditional<sizeof(ACC) == 8, double2, float2>::type;
+        const P2* pp = reinterpret_cast<const P2*>(prod + lane * K);
 #pragma unroll
-    for (int k = 0; k < K; ++k) p[k] = prod[k * L::kStride + lane];  // all reads before the first add
-    int rw[K];
-    const int nv = total - lane * K;  // valid slots of this lane (<= 0: none, >= K: all)
-    if constexpr (K == 8) {
-        unsigned long long word;
-        __builtin_memcpy(&word, row + lane * 8, 8);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) rw[k] = k < nv ? (int)((word >> (8 * k)) & 0xffu) : kNone;
+        for (int k = 0; k < K / 2; ++k) {
+            const P2 v = pp[k];
+            p[2 * k] = v.x;
+            p[2 * k + 1] = v.y;
+        }
     } else {
 #pragma unroll
-        for (int k = 0; k < K; ++k) rw[k] = k < nv ? (int)row[lane * K + k] : kNone;
+        for (int k = 0; k < K; ++k) p[k] = prod[lane * K + k];
     }
-    // Slots past the end form a suffix, so a run is only ever closed on a
-    // real row.  Runs that end inside the lane go to LDS at once; the lane's
-    // last run is first summed with the last runs of neighbouring lanes on
-    // the same row (DPP segmented sum inside 16-lane rows), so that a row
-    // spread over many lanes costs one LDS add, not one per lane.  Runs past
-    // the end (row kNone) are dropped.
+    int rw[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) rw[k] = row[lane * K + k];
+    // Positions past the end form a suffix, so a run is only ever closed on
+    // a real row.  Runs that end inside the lane go to LDS at once; the
+    // lane's last run is first summed with the last runs of neighbouring
+    // lanes on the same row (DPP segmented sum inside 16-lane rows), so that
+    // a row spread over many lanes costs one LDS add, not one per lane.
+    // Runs past the end (row kNoRow) are dropped.
     ACC s = p[0];
     int rp = rw[0];
 #pragma unroll
@@ -246,20 +367,20 @@ __device__ __forceinline__ void reduce_slots(ACC* acc, const unsigned char* row,
         rp = rw[k];
     }
     const bool tail = row_segmented_sum(s, rp);
-    if (tail && rp != kNone) atomicAdd(&acc[rp], s);
+    if (tail && rp != kNoRow) atomicAdd(&acc[rp], s);
 }
 
-template <class ACC, int K>
-__device__ __forceinline__ void accumulate_dpp(ACC* acc, int total, const ACC* a, const ACC* xv, const int* rw) {
+// Marks the lane's K positions as past the batch's end (before the next
+// batch's expansion overwrites the taken ones).
+template <int K>
+__device__ __forceinline__ void clear_rows(unsigned char* row, int lane) {
+    if constexpr (K % 2 == 0) {
+        unsigned short* rr = reinterpret_cast<unsigned short*>(row + lane * K);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        if (64 * k >= total) break;  // wave-uniform
-        const bool ok = rw[k] >= 0;
-        // Runs of one row in adjacent lanes are summed first (DPP), so one
-        // LDS atomic per run replaces one per value.
-        ACC pr = ok ? a[k] * xv[k] : (ACC)0;
-        const bool tail = row_segmented_sum(pr, rw[k]);
-        if (ok && tail) atomicAdd(&acc[rw[k]], pr);
+        for (int k = 0; k < K / 2; ++k) rr[k] = (unsigned short)(kNoRow * 0x101);
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) row[lane * K + k] = (unsigned char)kNoRow;
     }
 }
 
@@ -340,42 +461,116 @@ __device__ __forceinline__ BatchState<NW> take_batch(const HeaderRegs<R, MB>& h,
     return st;
 }
 
+// Values of one batch through a buffer resource based at its first value
+// (v, wave-uniform) and `total` values long: lane offsets are constants,
+// and the hardware range check returns 0 for the positions past the
+// batch's end without reading them.
+template <class V>
+struct ValueBuf;
+template <>
+struct ValueBuf<double> {
+    __device__ __forceinline__ static double get(__amdgpu_buffer_rsrc_t r, int off) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, kNonTemporal));
+    }
+};
+template <>
+struct ValueBuf<float> {
+    __device__ __forceinline__ static float get(__amdgpu_buffer_rsrc_t r, int off) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, kNonTemporal));
+    }
+};
+template <>
+struct ValueBuf<unsigned short> {
+    __device__ __forceinline__ static float get(__amdgpu_buffer_rsrc_t r, int off) {
+        return bf16_to_f32(__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, kNonTemporal));
+    }
+};
+
+template <class V, class ACC, int K>
+__device__ __forceinline__ void load_values_buf(const V* __restrict__ val, int v, int total, int lane, ACC* a) {
+    const int vs = __builtin_amdgcn_readfirstlane(v);
+    const int nrec = __builtin_amdgcn_readfirstlane(total) * (int)sizeof(V);  // total <= kExpandCap
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<V*>(val + vs), (short)0, nrec, kRawBufferFlags);
+    const int tot = __builtin_amdgcn_readfirstlane(total);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        // Chunks wholly past the batch's end are not issued (uniform branch).
+        a[k] = (ACC)0;
+        if (64 * k < tot) a[k] = (ACC)ValueBuf<V>::get(r, (lane + 64 * k) * (int)sizeof(V));
+    }
+}
+
 // x for every value of one expanded batch: columns of every chunk first
-// (past the batch's end: column 0, a valid x entry whose product with the
-// zero value is unused), then all K requests.  No branch, so that nothing
-// waits between the requests.
-template <class X, class ACC, int K, class SM>
-__device__ __forceinline__ void request_x(SM& sm, const X* __restrict__ x, int total, int lane, ACC* xv) {
+// (past the batch's end: column 0, a valid x entry whose product is never
+// added), then all K requests.  No branch, so that nothing waits between
+// the requests.  kWideX false: the LDS entries are byte offsets from x
+// (x spans less than 4 GiB), read as 32-bit offsets from the x base.
+// slot[k] is the position of the product of value lane + 64k (row-sorted
+// batches: its row-sorted position; past the batch's end: its own, a
+// position no row sum reads).
+template <class X, class ACC, int K, bool kSorted, bool kWideX, class SM>
+__device__ __forceinline__ void request_x(SM& sm, const X* __restrict__ x, int total, int lane, ACC* xv,
+                                          int* slot) {
     int cl[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int q = lane + 64 * k;
-        const int c = sm.col[q];
-        cl[k] = q < total ? c : 0;
+        if constexpr (kSorted) {
+            const int2 e = sm.cp[q];
+            cl[k] = q < total ? e.x : 0;
+            slot[k] = q < total ? e.y : q;
+        } else {
+            const int c = sm.col[q];
+            cl[k] = q < total ? c : 0;
+            slot[k] = q;
+        }
     }
     wave_sync();  // every column is read before products overwrite them
+    const int tot = __builtin_amdgcn_readfirstlane(total);
 #pragma unroll
-    for (int k = 0; k < K; ++k) xv[k] = (ACC)x[cl[k]];
+    for (int k = 0; k < K; ++k) {
+        xv[k] = (ACC)0;
+        if (64 * k >= tot) continue;  // chunk wholly past the batch's end (uniform)
+        if constexpr (kWideX) {
+            xv[k] = (ACC)x[cl[k]];
+        } else {
+            xv[k] = (ACC)*reinterpret_cast<const X*>(reinterpret_cast<const char*>(x) + (unsigned)cl[k]);
+        }
+    }
 }
 
-// Products of one batch (values a, x xv) and their row sums into the
-// tile's accumulators.  Chunks past the batch's end store unused products.
+// Products of one batch (values a, x xv) at their positions, then the row
+// sums into the tile's accumulators.
 template <class ACC, int K, class SM>
-__device__ __forceinline__ void reduce_batch(SM& sm, const ACC* a, const ACC* xv, int total, int lane) {
+__device__ __forceinline__ void reduce_batch(SM& sm, const ACC* a, const ACC* xv, const int* slot, int total,
+                                             int lane) {
+    const int tot = __builtin_amdgcn_readfirstlane(total);
 #pragma unroll
-    for (int k = 0; k < K; ++k) sm.prod[SlotLayout<ACC>::pos(lane + 64 * k)] = a[k] * xv[k];
+    for (int k = 0; k < K; ++k) {
+        if (64 * k < tot) sm.prod[slot[k]] = a[k] * xv[k];  // chunks past the end are never read as a run
+    }
     wave_sync();
-    reduce_slots<ACC, K>(sm.acc, sm.row, sm.prod, total, lane);
+    reduce_slots<ACC, K>(sm.acc, sm.row, sm.prod, lane);
+    clear_rows<K>(sm.row, lane);
 }
 
-template <class V, class X, class ACC, int R, int VS, int MB>
-__device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __restrict__ bptr,
-                                          const unsigned char* __restrict__ hdr, const V* __restrict__ val,
-                                          const X* __restrict__ x, X* __restrict__ y, int br0, int br1, int vstart,
-                                          int vend, int64_t nrows, int64_t ncols, const Epilogue& ep) {
-    constexpr int K = kExpandCap / 64;
+template <class V, class X, class ACC, int R, int VS, int MB, bool kSort, bool kWideX>
+__device__ __forceinline__ void spmv_tile_body(TileSmem<ACC, R, kSort>& sm, const int* __restrict__ bptr,
+                                               const unsigned char* __restrict__ hdr, const V* __restrict__ val,
+                                               const X* __restrict__ x, X* __restrict__ y, int br0, int br1,
+                                               int vstart, int vend, int64_t nrows, const Epilogue& ep) {
+    constexpr int K = kSlots;
     constexpr int NW = MaskWords<R * MB>::kWords;
+    constexpr bool kSorted = RowSort<R, kSort>::kOn;
+    constexpr int kScale = kWideX ? 1 : (int)sizeof(X);  // LDS column entries: indices or byte offsets
     const int lane = lane_id();
+    // The tile bounds are the same on every lane (the tile index comes from
+    // the wave index, which the compiler cannot tell is uniform).
+    br0 = __builtin_amdgcn_readfirstlane(br0);
+    br1 = __builtin_amdgcn_readfirstlane(br1);
+    vstart = __builtin_amdgcn_readfirstlane(vstart);
+    vend = __builtin_amdgcn_readfirstlane(vend);
 
     const int nbr = br1 - br0;
     const int b0 = bptr[br0];
@@ -387,6 +582,7 @@ __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __res
 
     for (int i = lane; i <= nbr; i += 64) sm.rs[i] = bptr[br0 + i] - b0;
     for (int i = lane; i < nbr * R; i += 64) sm.acc[i] = (ACC)0;
+    clear_rows<K>(sm.row, lane);
     int top = 1;
     while (top < nbr) top <<= 1;
 
@@ -399,28 +595,33 @@ __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __res
     while (b < b1) {
         const BatchState<NW> cur = take_batch<R, MB, NW>(h, b, b1, lane);
         ACC a[K];
+#if SPC5_VALUE_BUF
+        load_values_buf<V, ACC, K>(val, v, cur.bs.total, lane, a);
+#else
         load_values<V, ACC, K>(val, v, cur.bs.total, lane, a);
+#endif
         const int bn = b + cur.bs.ntake;
         load_header_raw<R, MB>(hdr, bn + lane, b1 - 1, h);  // past the end: block b1-1, masked
         const int br = batch_blockrows(sm.rs, sm.mark, nbr, top, b - b0, cur.bs.ntake, prev, lane);
         prev = __builtin_amdgcn_readlane(br, cur.bs.ntake - 1);
-        if (lane < cur.bs.ntake)
-            expand_rows<R, VS, NW>(sm.col, sm.row, br * R, cur.cs, cur.mw, cur.bs.incl - cur.bs.cnt);
+        // First column of the block, as an index or as a byte offset from x.
+        const int cso = (int)((unsigned)cur.cs * (unsigned)kScale);
+        if constexpr (kSorted) {
+            int d[R];
+            row_displacements<R, VS, NW>(sm.segs, cur.mw, br, cur.bs.ntake, cur.bs.incl - cur.bs.cnt, lane, d);
+            if (lane < cur.bs.ntake)
+                expand_rows_sorted<R, VS, NW, kScale>(sm.cp, sm.row, br * R, cso, cur.mw,
+                                                      cur.bs.incl - cur.bs.cnt, d);
+        } else {
+            if (lane < cur.bs.ntake)
+                expand_rows<R, VS, NW, kScale>(sm.col, sm.row, br * R, cso, cur.mw, cur.bs.incl - cur.bs.cnt);
+        }
         wave_sync();
         if (cur.bs.total > 0) {
             ACC xv[K];
-            request_x<X, ACC, K>(sm, x, cur.bs.total, lane, xv);
-#if SPC5_REDUCE
-            reduce_batch<ACC, K>(sm, a, xv, cur.bs.total, lane);
-#else
-            int rw[K];
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const int q = lane + 64 * k;
-                rw[k] = q < cur.bs.total ? (int)sm.row[q] : -1;
-            }
-            accumulate_dpp<ACC, K>(sm.acc, cur.bs.total, a, xv, rw);
-#endif
+            int slot[K];
+            request_x<X, ACC, K, kSorted, kWideX>(sm, x, cur.bs.total, lane, xv, slot);
+            reduce_batch<ACC, K>(sm, a, xv, slot, cur.bs.total, lane);
         }
         v += cur.bs.total;
         b = bn;
@@ -442,6 +643,21 @@ __device__ __forceinline__ void spmv_tile(TileSmem<ACC, R>& sm, const int* __res
         part = wave_sum_f64(part);
         if (lane == 0) ep.dot[0] = part;
     }
+}
+
+// One tile; x is addressed by 32-bit byte offsets unless it spans 4 GiB or
+// more (ncols * sizeof(X)), a uniform choice per launch.
+template <class V, class X, class ACC, int R, int VS, int MB, bool kSort = false>
+__device__ __forceinline__ void spmv_tile(TileSmem<ACC, R, kSort>& sm, const int* __restrict__ bptr,
+                                          const unsigned char* __restrict__ hdr, const V* __restrict__ val,
+                                          const X* __restrict__ x, X* __restrict__ y, int br0, int br1, int vstart,
+                                          int vend, int64_t nrows, int64_t ncols, const Epilogue& ep) {
+    if (SPC5_X32 && (uint64_t)ncols * sizeof(X) <= 0xffffffffull)
+        spmv_tile_body<V, X, ACC, R, VS, MB, kSort, false>(sm, bptr, hdr, val, x, y, br0, br1, vstart, vend, nrows,
+                                                           ep);
+    else
+        spmv_tile_body<V, X, ACC, R, VS, MB, kSort, true>(sm, bptr, hdr, val, x, y, br0, br1, vstart, vend, nrows,
+                                                          ep);
 }
 
 }  // namespace spc5

@@ -11,7 +11,7 @@ import ctypes as C
 
 import torch
 
-from ._lib import SPC5_HIP_BF16, SPC5_HIP_F32, SPC5_HIP_F64, Spc5Error, check, lib
+from ._lib import SPC5_HIP_BF16, SPC5_HIP_F32, SPC5_HIP_F64, SPC5_HIP_OPT_ROW_SORT, Spc5Error, check, lib
 
 _VT = {torch.float64: SPC5_HIP_F64, torch.float32: SPC5_HIP_F32, torch.bfloat16: SPC5_HIP_BF16}
 _VEC_DTYPE = {SPC5_HIP_F64: torch.float64, SPC5_HIP_F32: torch.float32, SPC5_HIP_BF16: torch.float32}
@@ -134,6 +134,17 @@ class DeviceMatrix:
         nt, nbr, nb, nnz = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
         check(lib().spc5_hip_mat_info(self._h, C.byref(nt), C.byref(nbr), C.byref(nb), C.byref(nnz)), "info")
         return {"ntiles": nt.value, "nblockrows": nbr.value, "nblocks": nb.value, "nnz": nnz.value}
+
+    @property
+    def row_sort(self) -> bool:
+        """Sum each batch's products in row order (r > 1; see
+        SPC5_HIP_OPT_ROW_SORT in include/spc5_hip.h)."""
+        return getattr(self, "_row_sort", False)
+
+    @row_sort.setter
+    def row_sort(self, on: bool):
+        check(lib().spc5_hip_mat_set_option(self._h, SPC5_HIP_OPT_ROW_SORT, 1 if on else 0), "set_option")
+        self._row_sort = bool(on)
 
     @property
     def format_name(self):

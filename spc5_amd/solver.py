@@ -74,9 +74,10 @@ def mixed_cg(A64: DeviceMatrix, Alo: DeviceMatrix, b, x=None, rtol=1e-10, delta=
     bfloat16 values, float products and sums); its vectors are float
     (vectors="f32", with a float correction folded into x at every update)
     or double ("f64").  b - A64 x is recomputed in double whenever the
-    iterated residual has dropped by `delta`.  Below tail_rtol (default 1e-6
-    for float values, 1e-3 for bfloat16; 0 = never) the solve finishes as
-    double CG with A64 from the same search direction."""
+    iterated residual has dropped by `delta`.  Below tail_rtol (default: 0 =
+    never for float values, whose iteration converges to 1e-10 with the
+    reliable updates alone; 1e-3 for bfloat16, whose iteration stalls) the
+    solve finishes as double CG with A64 from the same search direction."""
     _need_cuda(b, "b", torch.float64)
     if x is None:
         x = torch.zeros_like(b)
@@ -84,7 +85,7 @@ def mixed_cg(A64: DeviceMatrix, Alo: DeviceMatrix, b, x=None, rtol=1e-10, delta=
     if vectors not in ("f64", "f32"):
         raise Spc5Error("vectors must be 'f64' or 'f32'")
     if tail_rtol is None:
-        tail_rtol = 1e-3 if Alo.values.dtype == torch.bfloat16 else 1e-6
+        tail_rtol = 1e-3 if Alo.values.dtype == torch.bfloat16 else 0.0
     st = SolveStats()
     check(lib().spc5_hip_mixed_cg(A64._h, Alo._h, _ptr(b), _ptr(x), float(rtol), float(delta), int(maxit),
                                   1 if vectors == "f64" else 0, float(tail_rtol), C.byref(st), stream_ptr(stream)),

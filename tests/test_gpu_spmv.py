@@ -45,13 +45,14 @@ def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).view(np.uint8).tobytes()).hexdigest()
 
 
-def gpu_spmv_from_arrays(r, n, ncols, bptr, hdr, val, x, y0, vs=None):
+def gpu_spmv_from_arrays(r, n, ncols, bptr, hdr, val, x, y0, vs=None, row_sort=False):
     from spc5_amd.device import DeviceMatrix
     nb = int(bptr[(n + r - 1) // r])
     vs = O.VS_OF[np.dtype(val.dtype)] if vs is None else vs
     m = DeviceMatrix(r=r, vs=vs, nrows=n, ncols=ncols, nnz=len(val), nblocks=nb,
                      rows_size=dev(np.ascontiguousarray(bptr[: (n + r - 1) // r + 1], np.int32)),
                      headers=dev(hdr.astype(np.uint8)), values=dev(val))
+    m.row_sort = row_sort
     xd = dev(x[:ncols])
     yd = dev(y0[:n])
     m.spmv(xd, yd)
@@ -152,7 +153,8 @@ def _oracle_case(rp, col, v, r, seed, ncols):
 @pytest.mark.parametrize("tag", ["f64", "f32"])
 @pytest.mark.parametrize("r", RS)
 @pytest.mark.parametrize("shape", ["rand", "clustered", "tall", "wide"])
-def test_gpu_spmv_vs_oracle(seed, tag, r, shape):
+@pytest.mark.parametrize("row_sort", [False, True])
+def test_gpu_spmv_vs_oracle(seed, tag, r, shape, row_sort):
     dt = DT[tag]
     if shape == "rand":
         rp, col, v = random_csr(333 + seed, 500, 0.02, seed, dt)
@@ -168,13 +170,14 @@ def test_gpu_spmv_vs_oracle(seed, tag, r, shape):
         ncols = 20000
     n = len(rp) - 1
     x, y0, bptr, hdr, vout, yref = _oracle_case(rp, col, v, r, seed, ncols)
-    y = gpu_spmv_from_arrays(r, n, ncols, bptr, hdr, vout, x, y0)
+    y = gpu_spmv_from_arrays(r, n, ncols, bptr, hdr, vout, x, y0, row_sort=row_sort)
     assert normwise_error(y, yref[:n], rp, v, x[:ncols], y0[:n]) <= TOL[tag]
 
 
 @pytest.mark.parametrize("tag", ["f64", "f32"])
 @pytest.mark.parametrize("r", RS)
-def test_gpu_long_and_dense_rows(tag, r):
+@pytest.mark.parametrize("row_sort", [False, True])
+def test_gpu_long_and_dense_rows(tag, r, row_sort):
     """Block-rows far longer than a tile's value target and than one LDS
     batch; full masks (64 or 128 values per block)."""
     dt = DT[tag]
@@ -196,7 +199,7 @@ def test_gpu_long_and_dense_rows(tag, r):
     col = np.concatenate(sets).astype(np.int32)
     v = rng.uniform(-1, 1, len(col)).astype(dt)
     x, y0, bptr, hdr, vout, yref = _oracle_case(rowptr, col, v, r, r, ncols)
-    y = gpu_spmv_from_arrays(r, n, ncols, bptr, hdr, vout, x, y0)
+    y = gpu_spmv_from_arrays(r, n, ncols, bptr, hdr, vout, x, y0, row_sort=row_sort)
     assert normwise_error(y, yref[:n], rowptr, v, x[:ncols], y0[:n]) <= TOL[tag]
 
 
@@ -303,3 +306,18 @@ def test_clustered_runs_full_size_properties(r):
         assert abs(yh[i] - exact) <= 1e-12 * max(1.0, float(np.abs(v[s:e]).sum()))
     del m
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("tag", ["f64", "f32"])
+@pytest.mark.parametrize("r", RS)
+@pytest.mark.parametrize("row_sort", [False, True])
+def test_gpu_stencil27(tag, r, row_sort):
+    """27-point stencil: every block of a block-row covers the same r rows,
+    the structure the row-sorted batches are for."""
+    from spc5_amd import synth
+    rp, col, v = synth.stencil27(20)
+    v = v.astype(DT[tag])
+    n = len(rp) - 1
+    x, y0, bptr, hdr, vout, yref = _oracle_case(rp, col, v, r, 7, n)
+    y = gpu_spmv_from_arrays(r, n, n, bptr, hdr, vout, x, y0, row_sort=row_sort)
+    assert normwise_error(y, yref[:n], rp, v, x[:n], y0[:n]) <= TOL[tag]
