@@ -129,7 +129,7 @@ int spc5_hip_mat_info(spc5_hip_mat_t h, int64_t* ntiles, int64_t* nblockrows, in
                       int64_t* nnz);
 
 /* Per-handle kernel options (spc5_hip_mat_set_option).
- * ROW_SORT (0 or 1, default 0; r > 1 only): the products of every batch of
+ * ROW_SORT (0 or 1, default 0; used for r = 2 and 4): the products of every batch of
  * blocks are summed in row order instead of block order.  Pays off where
  * every block of a block-row covers the same rows (stencils: all blocks of
  * a block-row add to the same r rows at the same step), costs extra work

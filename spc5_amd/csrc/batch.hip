@@ -53,12 +53,12 @@ struct spc5_hip_batch {
 namespace spc5 {
 namespace {
 
-template <class X, class ACC, int VS>
+template <class V, class ACC>
 union BatchSmem {
-    TileSmem<ACC, 1> r1;
-    TileSmem<ACC, 2> r2;
-    TileSmem<ACC, 4> r4;
-    TileSmem<ACC, 8> r8;
+    TileSmem<ACC, 1, expand_cap<V>()> r1;
+    TileSmem<ACC, 2, expand_cap<V>()> r2;
+    TileSmem<ACC, 4, expand_cap<V>()> r4;
+    TileSmem<ACC, 8, expand_cap<V>()> r8;
 };
 #define SPC5_BATCH_TILE(RR, SMEM) \
     spmv_tile<V, X, ACC, RR, VS, MB>(SMEM, m.bptr, m.hdr, val, x, y, br0, br1, v0, v1, m.nrows, m.ncols, ep)
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_batch(const BatchE
                                                                      const int* __restrict__ tile_mat,
                                                                      const int* __restrict__ tile_loc,
                                                                      int64_t ntiles) {
-    __shared__ BatchSmem<X, ACC, VS> sm[kWavesPerGroup];
+    __shared__ BatchSmem<V, ACC> sm[kWavesPerGroup];
     const int w = threadIdx.x >> 6;
     const int64_t stride = (int64_t)gridDim.x * kWavesPerGroup;
     const Epilogue ep;

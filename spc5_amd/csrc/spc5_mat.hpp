@@ -20,7 +20,10 @@ namespace spc5 {
 #define SPC5_WAVES_PER_GROUP 1
 #endif
 #ifndef SPC5_EXPAND_CAP
-#define SPC5_EXPAND_CAP 384
+#define SPC5_EXPAND_CAP 384  // values expanded into LDS per batch, 8-byte values
+#endif
+#ifndef SPC5_EXPAND_CAP_NARROW
+#define SPC5_EXPAND_CAP_NARROW 512  // same, 4- and 2-byte values (fixed costs weigh more per byte)
 #endif
 #ifndef SPC5_SEGREDUCE
 #define SPC5_SEGREDUCE 1
@@ -40,9 +43,16 @@ using FloatAcc = float;
 #endif
 constexpr int kMaxRows = SPC5_MAX_ROWS;              // scalar rows per tile (LDS accumulators), <= 256
 constexpr int kWavesPerGroup = SPC5_WAVES_PER_GROUP; // independent waves per workgroup
-constexpr int kExpandCap = SPC5_EXPAND_CAP;          // values expanded into LDS per batch
 static_assert(kMaxRows <= 255, "tile-local rows are stored as bytes, kMaxRows marks none");
-static_assert(kExpandCap % 64 == 0 && kExpandCap >= 128, "expansion cap must hold a full block");
+// Values expanded into LDS per batch for value type V.
+template <class V>
+constexpr int expand_cap() {
+    return sizeof(V) == 8 ? SPC5_EXPAND_CAP : SPC5_EXPAND_CAP_NARROW;
+}
+static_assert(SPC5_EXPAND_CAP % 64 == 0 && SPC5_EXPAND_CAP >= 128 && SPC5_EXPAND_CAP < 1024,
+              "expansion cap: whole chunks, a full block, 10-bit counts");
+static_assert(SPC5_EXPAND_CAP_NARROW % 64 == 0 && SPC5_EXPAND_CAP_NARROW >= 128 && SPC5_EXPAND_CAP_NARROW < 1024,
+              "expansion cap: whole chunks, a full block, 10-bit counts");
 
 // Output stage of a product: y = alpha * (A x) + beta * y (y is not read
 // when beta == 0); with dv != null, tile t writes sum(dv[i] * y_new[i]) over

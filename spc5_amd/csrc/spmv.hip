@@ -8,7 +8,7 @@
 //   * batch step: lane i decodes block b+i (unaligned int32 column + r
 //     masks), counts its values (popcount), and a DPP wave scan gives each
 //     block's position in the batch; blocks are taken while the batch holds
-//     at most kExpandCap values;
+//     at most expand_cap<V>() values;
 //   * each taken lane finds its block-row by binary search in LDS and
 //     expands its set bits into (column, local row) pairs in LDS -- values
 //     are never zero-padded;
@@ -43,18 +43,22 @@ __device__ __forceinline__ int64_t xcd_tile(int wave) {
     return gx * kWavesPerGroup + wave;
 }
 
-// SPC5_WAVES_PER_EU (1: compiler's choice) asks for at least that many
-// waves per SIMD, i.e. a register budget of 512 / waves VGPRs.
+// Waves per SIMD each tile kernel is compiled for (a register budget of
+// 512 / waves VGPRs): the most that compile without spills.  0 leaves the
+// choice to the compiler.
 #ifndef SPC5_WAVES_PER_EU
-#define SPC5_WAVES_PER_EU 1
+#define SPC5_WAVES_PER_EU 8
 #endif
+constexpr int tile_waves_per_eu(int r, bool sorted) {
+    return SPC5_WAVES_PER_EU == 0 ? 1 : sorted ? (r == 2 ? 6 : 5) : (r == 8 ? 7 : SPC5_WAVES_PER_EU);
+}
 
 template <class V, class X, class ACC, int R, int VS, int MB, bool SORT>
-__global__ __launch_bounds__(kWavesPerGroup * 64, SPC5_WAVES_PER_EU) void k_spmv_tiles(
+__global__ __launch_bounds__(kWavesPerGroup * 64, tile_waves_per_eu(R, SORT)) void k_spmv_tiles(
     const int* __restrict__ bptr, const unsigned char* __restrict__ hdr, const V* __restrict__ val,
     const X* __restrict__ x, X* __restrict__ y, const int* __restrict__ tile_br,
     const int* __restrict__ tile_val, int64_t ntiles, int64_t nrows, int64_t ncols, Epilogue ep) {
-    __shared__ TileSmem<ACC, R, SORT> sm[kWavesPerGroup];
+    __shared__ TileSmem<ACC, R, expand_cap<V>(), SORT> sm[kWavesPerGroup];
     const int w = threadIdx.x >> 6;
     const int64_t t = xcd_tile(w);
     if (t >= ntiles) return;  // whole wave leaves; no workgroup barrier is used
@@ -165,10 +169,7 @@ int launch_spmv_vs(const spc5_hip_mat& m, const void* x, void* y, hipStream_t s,
         if (sort) launch_tiles<V, X, ACC, 4, VS, MB, true>(m, x, y, s, ep);
         else launch_tiles<V, X, ACC, 4, VS, MB, false>(m, x, y, s, ep);
         break;
-    case 8:
-        if (sort) launch_tiles<V, X, ACC, 8, VS, MB, true>(m, x, y, s, ep);
-        else launch_tiles<V, X, ACC, 8, VS, MB, false>(m, x, y, s, ep);
-        break;
+    case 8: launch_tiles<V, X, ACC, 8, VS, MB, false>(m, x, y, s, ep); break;  // row sort: r = 2, 4
     default:
         set_error("invalid r");
         return SPC5_HIP_ERR_INVALID;

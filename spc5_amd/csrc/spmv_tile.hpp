@@ -5,8 +5,8 @@
 //   1. the lane's block header (int32 first column + r masks) was loaded
 //      ahead; popcounts and a DPP wave scan give every block's value offset
 //      inside the batch; blocks are taken while the batch holds at most
-//      kExpandCap values;
-//   2. the batch's values are loaded at once, kExpandCap/64 coalesced loads
+//      expand_cap<V>() values;
+//   2. the batch's values are loaded at once, expand_cap<V>()/64 coalesced loads
 //      per lane, straight from the value stream (no zero padding), through
 //      a buffer resource based at the batch's first value (constant lane
 //      offsets; the range check returns 0 past the tile's values);
@@ -49,23 +49,23 @@ struct RowSort {
     static constexpr int kWords = (R + 2) / 3;  // packed words of three 10-bit row counts
 };
 
-constexpr int kSlots = kExpandCap / 64;  // batch positions owned by one lane in the row sums
-constexpr int kNoRow = kMaxRows;         // row byte of the positions past the batch's end
+constexpr int kNoRow = kMaxRows;  // row byte of the positions past the batch's end
 
 
-template <class ACC, int R, bool kSort = false>
+// LDS of one wave's tile: CAP values per batch.
+template <class ACC, int R, int CAP, bool kSort = false>
 struct TileSmem {
     ACC acc[kMaxRows];  // row sums of the tile
     // Columns in value order; every column is read before the first
     // product is stored over them.
     // (x byte offsets instead of columns when x spans less than 4 GiB).
     union alignas(16) {
-        int col[kExpandCap];
-        int2 cp[RowSort<R, kSort>::kOn ? kExpandCap : 1];  // row-sorted: column and row-sorted position
-        ACC prod[kExpandCap];                       // products at their (row-sorted) positions
+        int col[CAP];
+        int2 cp[RowSort<R, kSort>::kOn ? CAP : 1];  // row-sorted: column and row-sorted position
+        ACC prod[CAP];                              // products at their (row-sorted) positions
     };
     // Tile-local row of batch position q; kNoRow past the batch's end.
-    alignas(8) unsigned char row[kExpandCap];
+    alignas(8) unsigned char row[CAP];
     int rs[kMaxRows / R + 1];
     int mark[64];  // block-row marks of one batch
     int segs[RowSort<R, kSort>::kOn ? 2 * RowSort<R, kSort>::kWords * 64 : 1];  // row counts at segment ends
@@ -121,14 +121,14 @@ struct BatchShape {
     int total;  // values taken in this batch
 };
 
-template <int NW>
+template <int NW, int CAP>
 __device__ __forceinline__ BatchShape batch_shape(const unsigned long long* mw, bool valid) {
     BatchShape bs;
     bs.cnt = 0;
 #pragma unroll
     for (int k = 0; k < NW; ++k) bs.cnt += __popcll(mw[k]);
     bs.incl = wave_incl_scan(bs.cnt);
-    bs.ntake = __popcll(ballot64(valid && bs.incl <= kExpandCap));
+    bs.ntake = __popcll(ballot64(valid && bs.incl <= CAP));
     bs.total = bs.ntake > 0 ? __builtin_amdgcn_readlane(bs.incl, bs.ntake - 1) : 0;
     return bs;
 }
@@ -208,15 +208,13 @@ __device__ __forceinline__ int row_count(const unsigned long long* mw, int t) {
 // the lane's block.
 //
 // Per-row counts of the taken blocks are packed three to a word (10-bit
-// fields; a batch holds at most kExpandCap < 1024 values, so no field
-// carries) and prefix-summed over the wave; the first and last lane of
+// fields; a batch holds fewer than 1024 values, so no field carries) and prefix-summed over the wave; the first and last lane of
 // every block-row segment leave the prefix at the segment's start and end
 // in LDS, from which every lane gets its segment's row totals.
 template <int R, int VS, int NW>
 __device__ __forceinline__ void row_displacements(int* segs, const unsigned long long* mw, int br, int ntake, int p0,
                                                   int lane, int* d) {
     constexpr int kW = RowSort<R, true>::kWords;
-    static_assert(kExpandCap < 1024, "10-bit row-count fields");
     const bool taken = lane < ntake;
     int c[R];
     int pk[kW] = {};
@@ -278,20 +276,20 @@ __device__ __forceinline__ void expand_rows_sorted(int2* cp, unsigned char* row,
             }
         }
     } else {
+        // One loop per block row: the row and its displacement are
+        // compile-time indices (no dynamically indexed array).
 #pragma unroll
-        for (int wd = 0; wd < NW; ++wd) {
-            unsigned long long m = mw[wd];
+        for (int t = 0; t < R; ++t) {
+            const int bit0 = t * VS;
+            unsigned m = (unsigned)((mw[bit0 / 64] >> (bit0 % 64)) & ((1ull << VS) - 1ull));
+            const int dt = d[t];
+            const unsigned char rt = (unsigned char)(rbase + t);
             while (m) {
-                const int bit = __builtin_ctzll(m) + wd * 64;
-                const int t = bit / VS;
-                int dt = d[0];
-#pragma unroll
-                for (int u = 1; u < R; ++u) dt = t == u ? d[u] : dt;
-                const int s = p + dt;
-                cp[p] = make_int2(cs + (bit & (VS - 1)) * kScale, s);
-                row[s] = (unsigned char)(rbase + t);
+                const int c = __builtin_ctz(m);
+                cp[p] = make_int2(cs + c * kScale, p + dt);
+                row[p + dt] = rt;
                 ++p;
-                m &= m - 1ull;
+                m &= m - 1u;
             }
         }
     }
@@ -425,7 +423,7 @@ __device__ __forceinline__ void load_header_raw(const unsigned char* __restrict_
 // Shape of the batch starting at block b from its (prefetched) headers;
 // lanes past the tile's last block count as empty.  This is the first use
 // of the header registers, so their wait happens here.
-template <int R, int MB, int NW>
+template <int R, int MB, int NW, int CAP>
 __device__ __forceinline__ BatchState<NW> take_batch(const HeaderRegs<R, MB>& h, int b, int b1, int lane) {
     using H = HeaderRegs<R, MB>;
     BatchState<NW> st;
@@ -457,7 +455,7 @@ __device__ __forceinline__ BatchState<NW> take_batch(const HeaderRegs<R, MB>& h,
         st.mw[k] = valid ? w : 0ull;
     }
     st.cs = valid ? cs : 0;
-    st.bs = batch_shape<NW>(st.mw, valid);
+    st.bs = batch_shape<NW, CAP>(st.mw, valid);
     return st;
 }
 
@@ -489,14 +487,14 @@ struct ValueBuf<unsigned short> {
 template <class V, class ACC, int K>
 __device__ __forceinline__ void load_values_buf(const V* __restrict__ val, int v, int total, int lane, ACC* a) {
     const int vs = __builtin_amdgcn_readfirstlane(v);
-    const int nrec = __builtin_amdgcn_readfirstlane(total) * (int)sizeof(V);  // total <= kExpandCap
+    const int nrec = __builtin_amdgcn_readfirstlane(total) * (int)sizeof(V);  // total < 1024
     const __amdgpu_buffer_rsrc_t r =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<V*>(val + vs), (short)0, nrec, kRawBufferFlags);
     const int tot = __builtin_amdgcn_readfirstlane(total);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        // Chunks wholly past the batch's end are not issued (uniform branch).
-        a[k] = (ACC)0;
+        // Chunks wholly past the batch's end are not issued (uniform branch);
+        // their registers are never read (reduce_batch skips the same chunks).
         if (64 * k < tot) a[k] = (ACC)ValueBuf<V>::get(r, (lane + 64 * k) * (int)sizeof(V));
     }
 }
@@ -530,8 +528,7 @@ __device__ __forceinline__ void request_x(SM& sm, const X* __restrict__ x, int t
     const int tot = __builtin_amdgcn_readfirstlane(total);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        xv[k] = (ACC)0;
-        if (64 * k >= tot) continue;  // chunk wholly past the batch's end (uniform)
+        if (64 * k >= tot) continue;  // chunk wholly past the batch's end (uniform; never read)
         if constexpr (kWideX) {
             xv[k] = (ACC)x[cl[k]];
         } else {
@@ -556,11 +553,13 @@ __device__ __forceinline__ void reduce_batch(SM& sm, const ACC* a, const ACC* xv
 }
 
 template <class V, class X, class ACC, int R, int VS, int MB, bool kSort, bool kWideX>
-__device__ __forceinline__ void spmv_tile_body(TileSmem<ACC, R, kSort>& sm, const int* __restrict__ bptr,
+__device__ __forceinline__ void spmv_tile_body(TileSmem<ACC, R, expand_cap<V>(), kSort>& sm,
+                                               const int* __restrict__ bptr,
                                                const unsigned char* __restrict__ hdr, const V* __restrict__ val,
                                                const X* __restrict__ x, X* __restrict__ y, int br0, int br1,
                                                int vstart, int vend, int64_t nrows, const Epilogue& ep) {
-    constexpr int K = kSlots;
+    constexpr int CAP = expand_cap<V>();
+    constexpr int K = CAP / 64;  // batch positions owned by one lane in the row sums
     constexpr int NW = MaskWords<R * MB>::kWords;
     constexpr bool kSorted = RowSort<R, kSort>::kOn;
     constexpr int kScale = kWideX ? 1 : (int)sizeof(X);  // LDS column entries: indices or byte offsets
@@ -593,7 +592,7 @@ __device__ __forceinline__ void spmv_tile_body(TileSmem<ACC, R, kSort>& sm, cons
     if (b < b1) load_header_raw<R, MB>(hdr, b + lane, b1 - 1, h);
     wave_sync();
     while (b < b1) {
-        const BatchState<NW> cur = take_batch<R, MB, NW>(h, b, b1, lane);
+        const BatchState<NW> cur = take_batch<R, MB, NW, CAP>(h, b, b1, lane);
         ACC a[K];
 #if SPC5_VALUE_BUF
         load_values_buf<V, ACC, K>(val, v, cur.bs.total, lane, a);
@@ -648,7 +647,8 @@ __device__ __forceinline__ void spmv_tile_body(TileSmem<ACC, R, kSort>& sm, cons
 // One tile; x is addressed by 32-bit byte offsets unless it spans 4 GiB or
 // more (ncols * sizeof(X)), a uniform choice per launch.
 template <class V, class X, class ACC, int R, int VS, int MB, bool kSort = false>
-__device__ __forceinline__ void spmv_tile(TileSmem<ACC, R, kSort>& sm, const int* __restrict__ bptr,
+__device__ __forceinline__ void spmv_tile(TileSmem<ACC, R, expand_cap<V>(), kSort>& sm,
+                                          const int* __restrict__ bptr,
                                           const unsigned char* __restrict__ hdr, const V* __restrict__ val,
                                           const X* __restrict__ x, X* __restrict__ y, int br0, int br1, int vstart,
                                           int vend, int64_t nrows, int64_t ncols, const Epilogue& ep) {
