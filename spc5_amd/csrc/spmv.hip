@@ -49,12 +49,15 @@ __device__ __forceinline__ int64_t xcd_tile(int wave) {
 #ifndef SPC5_WAVES_PER_EU
 #define SPC5_WAVES_PER_EU 8
 #endif
-constexpr int tile_waves_per_eu(int r, bool sorted) {
-    return SPC5_WAVES_PER_EU == 0 ? 1 : sorted ? (r == 2 ? 6 : 5) : (r == 8 ? 7 : SPC5_WAVES_PER_EU);
+constexpr int tile_waves_per_eu(int r, bool sorted, bool wide_x) {
+    // wide_x: double x with float sums (twice the registers per x value)
+    return SPC5_WAVES_PER_EU == 0 ? 1
+                                  : (sorted ? (r == 2 ? 6 : 5) : (r == 8 ? 7 : SPC5_WAVES_PER_EU)) - (wide_x ? 1 : 0);
 }
 
 template <class V, class X, class ACC, int R, int VS, int MB, bool SORT>
-__global__ __launch_bounds__(kWavesPerGroup * 64, tile_waves_per_eu(R, SORT)) void k_spmv_tiles(
+__global__ __launch_bounds__(kWavesPerGroup * 64, tile_waves_per_eu(R, SORT, sizeof(X) > sizeof(ACC))) void
+k_spmv_tiles(
     const int* __restrict__ bptr, const unsigned char* __restrict__ hdr, const V* __restrict__ val,
     const X* __restrict__ x, X* __restrict__ y, const int* __restrict__ tile_br,
     const int* __restrict__ tile_val, int64_t ntiles, int64_t nrows, int64_t ncols, Epilogue ep) {

@@ -32,12 +32,8 @@
 
 namespace spc5 {
 
-// Value stream through per-batch buffer resources (1) or lane-masked
-// global loads (0); x through 32-bit byte offsets when it spans < 4 GiB (1)
-// or always through 64-bit addresses (0).
-#ifndef SPC5_VALUE_BUF
-#define SPC5_VALUE_BUF 1
-#endif
+// x through 32-bit byte offsets when it spans < 4 GiB (1) or always through
+// 64-bit addresses (0).
 #ifndef SPC5_X32
 #define SPC5_X32 1
 #endif
@@ -463,29 +459,45 @@ __device__ __forceinline__ BatchState<NW> take_batch(const HeaderRegs<R, MB>& h,
 // (v, wave-uniform) and `total` values long: lane offsets are constants,
 // and the hardware range check returns 0 for the positions past the
 // batch's end without reading them.
+// Raw value bits as loaded (Raw) and their conversion to the accumulation
+// type, done where the product is formed: a conversion next to the load
+// (inside the branch that skips chunks past the batch's end) would wait for
+// the load there.
 template <class V>
 struct ValueBuf;
 template <>
 struct ValueBuf<double> {
-    __device__ __forceinline__ static double get(__amdgpu_buffer_rsrc_t r, int off) {
+    using Raw = double;
+    __device__ __forceinline__ static Raw get(__amdgpu_buffer_rsrc_t r, int off) {
         return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, kNonTemporal));
     }
+    template <class ACC>
+    __device__ __forceinline__ static ACC acc(Raw v) { return (ACC)v; }
 };
 template <>
 struct ValueBuf<float> {
-    __device__ __forceinline__ static float get(__amdgpu_buffer_rsrc_t r, int off) {
+    using Raw = float;
+    __device__ __forceinline__ static Raw get(__amdgpu_buffer_rsrc_t r, int off) {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, kNonTemporal));
     }
+    template <class ACC>
+    __device__ __forceinline__ static ACC acc(Raw v) { return (ACC)v; }
 };
 template <>
 struct ValueBuf<unsigned short> {
-    __device__ __forceinline__ static float get(__amdgpu_buffer_rsrc_t r, int off) {
-        return bf16_to_f32(__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, kNonTemporal));
+    // One 32-bit register per value (the load zero-extends): 16-bit
+    // registers would be packed in pairs, and the packing waits for the load.
+    using Raw = unsigned int;
+    __device__ __forceinline__ static Raw get(__amdgpu_buffer_rsrc_t r, int off) {
+        return (unsigned int)__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, kNonTemporal);
     }
+    template <class ACC>
+    __device__ __forceinline__ static ACC acc(Raw v) { return (ACC)__uint_as_float(v << 16); }
 };
 
-template <class V, class ACC, int K>
-__device__ __forceinline__ void load_values_buf(const V* __restrict__ val, int v, int total, int lane, ACC* a) {
+template <class V, int K>
+__device__ __forceinline__ void load_values_buf(const V* __restrict__ val, int v, int total, int lane,
+                                                typename ValueBuf<V>::Raw* a) {
     const int vs = __builtin_amdgcn_readfirstlane(v);
     const int nrec = __builtin_amdgcn_readfirstlane(total) * (int)sizeof(V);  // total < 1024
     const __amdgpu_buffer_rsrc_t r =
@@ -495,7 +507,7 @@ __device__ __forceinline__ void load_values_buf(const V* __restrict__ val, int v
     for (int k = 0; k < K; ++k) {
         // Chunks wholly past the batch's end are not issued (uniform branch);
         // their registers are never read (reduce_batch skips the same chunks).
-        if (64 * k < tot) a[k] = (ACC)ValueBuf<V>::get(r, (lane + 64 * k) * (int)sizeof(V));
+        if (64 * k < tot) a[k] = ValueBuf<V>::get(r, (lane + 64 * k) * (int)sizeof(V));
     }
 }
 
@@ -507,9 +519,8 @@ __device__ __forceinline__ void load_values_buf(const V* __restrict__ val, int v
 // slot[k] is the position of the product of value lane + 64k (row-sorted
 // batches: its row-sorted position; past the batch's end: its own, a
 // position no row sum reads).
-template <class X, class ACC, int K, bool kSorted, bool kWideX, class SM>
-__device__ __forceinline__ void request_x(SM& sm, const X* __restrict__ x, int total, int lane, ACC* xv,
-                                          int* slot) {
+template <class X, int K, bool kSorted, bool kWideX, class SM>
+__device__ __forceinline__ void request_x(SM& sm, const X* __restrict__ x, int total, int lane, X* xv, int* slot) {
     int cl[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -530,22 +541,23 @@ __device__ __forceinline__ void request_x(SM& sm, const X* __restrict__ x, int t
     for (int k = 0; k < K; ++k) {
         if (64 * k >= tot) continue;  // chunk wholly past the batch's end (uniform; never read)
         if constexpr (kWideX) {
-            xv[k] = (ACC)x[cl[k]];
+            xv[k] = x[cl[k]];
         } else {
-            xv[k] = (ACC)*reinterpret_cast<const X*>(reinterpret_cast<const char*>(x) + (unsigned)cl[k]);
+            xv[k] = *reinterpret_cast<const X*>(reinterpret_cast<const char*>(x) + (unsigned)cl[k]);
         }
     }
 }
 
 // Products of one batch (values a, x xv) at their positions, then the row
 // sums into the tile's accumulators.
-template <class ACC, int K, class SM>
-__device__ __forceinline__ void reduce_batch(SM& sm, const ACC* a, const ACC* xv, const int* slot, int total,
-                                             int lane) {
+template <class V, class X, class ACC, int K, class SM>
+__device__ __forceinline__ void reduce_batch(SM& sm, const typename ValueBuf<V>::Raw* a, const X* xv,
+                                             const int* slot, int total, int lane) {
     const int tot = __builtin_amdgcn_readfirstlane(total);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        if (64 * k < tot) sm.prod[slot[k]] = a[k] * xv[k];  // chunks past the end are never read as a run
+        // chunks past the end are never read as a run
+        if (64 * k < tot) sm.prod[slot[k]] = ValueBuf<V>::template acc<ACC>(a[k]) * (ACC)xv[k];
     }
     wave_sync();
     reduce_slots<ACC, K>(sm.acc, sm.row, sm.prod, lane);
@@ -593,12 +605,8 @@ __device__ __forceinline__ void spmv_tile_body(TileSmem<ACC, R, expand_cap<V>(),
     wave_sync();
     while (b < b1) {
         const BatchState<NW> cur = take_batch<R, MB, NW, CAP>(h, b, b1, lane);
-        ACC a[K];
-#if SPC5_VALUE_BUF
-        load_values_buf<V, ACC, K>(val, v, cur.bs.total, lane, a);
-#else
-        load_values<V, ACC, K>(val, v, cur.bs.total, lane, a);
-#endif
+        typename ValueBuf<V>::Raw a[K];
+        load_values_buf<V, K>(val, v, cur.bs.total, lane, a);
         const int bn = b + cur.bs.ntake;
         load_header_raw<R, MB>(hdr, bn + lane, b1 - 1, h);  // past the end: block b1-1, masked
         const int br = batch_blockrows(sm.rs, sm.mark, nbr, top, b - b0, cur.bs.ntake, prev, lane);
@@ -617,10 +625,10 @@ __device__ __forceinline__ void spmv_tile_body(TileSmem<ACC, R, expand_cap<V>(),
         }
         wave_sync();
         if (cur.bs.total > 0) {
-            ACC xv[K];
+            X xv[K];  // converted where the products are formed
             int slot[K];
-            request_x<X, ACC, K, kSorted, kWideX>(sm, x, cur.bs.total, lane, xv, slot);
-            reduce_batch<ACC, K>(sm, a, xv, slot, cur.bs.total, lane);
+            request_x<X, K, kSorted, kWideX>(sm, x, cur.bs.total, lane, xv, slot);
+            reduce_batch<V, X, ACC, K>(sm, a, xv, slot, cur.bs.total, lane);
         }
         v += cur.bs.total;
         b = bn;

@@ -39,7 +39,8 @@ def stencil27_small(n=12, diag=26.001):
 
 
 @pytest.mark.parametrize("r", (1, 2, 4, 8))
-def test_lowp_values_and_spmv(r):
+@pytest.mark.parametrize("row_sort", [False, True])
+def test_lowp_values_and_spmv(r, row_sort):
     from spc5_amd.device import DeviceMatrix
     from spc5_amd.solver import lowp
     rp, col, val = clustered_csr(700, 900, seed=r, empty_frac=0.05)
@@ -49,6 +50,7 @@ def test_lowp_values_and_spmv(r):
     Ad = csr_dense(rp, col, val, 900)
     for kind, dt, tol_exact in (("f32", torch.float32, 1e-6), ("bf16", torch.bfloat16, 2.0 ** -8)):
         Alo = lowp(A64, kind)
+        Alo.row_sort = row_sort
         assert Alo.values.dtype == dt
         ref_bits = v64.to(dt)
         assert torch.equal(Alo.values.cpu().view(torch.int16 if dt == torch.bfloat16 else torch.int32),
@@ -69,6 +71,29 @@ def test_lowp_values_and_spmv(r):
         assert normwise_error(y.cpu().numpy(), yref, rp, val, xf, np.zeros(1)) <= 1e-5
         assert normwise_error(y.cpu().numpy(), Ad @ xf.astype(np.float64), rp, val, xf, np.zeros(1)) <= tol_exact
         del rounded
+
+
+@pytest.mark.parametrize("r", (2, 4))
+@pytest.mark.parametrize("kind", ["f32", "bf16"])
+@pytest.mark.parametrize("row_sort", [False, True])
+def test_lowp_stencil_row_sort(r, kind, row_sort):
+    """Float and bfloat16 copies of the 27-point stencil (the structure the
+    row-sorted batches are for): the product equals the rounded matrix's
+    to float accumulation error."""
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import lowp
+    rp, col, val = stencil27_small(16)
+    n = len(rp) - 1
+    A64 = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val), r)
+    Alo = lowp(A64, kind)
+    Alo.row_sort = row_sort
+    dt = torch.float32 if kind == "f32" else torch.bfloat16
+    rounded = torch.from_numpy(val).to(dt).to(torch.float64).numpy()  # CSR order, same rounding
+    xf = np.random.default_rng(r).uniform(-1, 1, n).astype(np.float32)
+    y = torch.zeros(n, dtype=torch.float32, device="cuda")
+    Alo.spmv(dev(xf), y)
+    yref = csr_dense(rp, col, rounded, n) @ xf.astype(np.float64)
+    assert normwise_error(y.cpu().numpy(), yref, rp, val, xf, np.zeros(1)) <= 1e-5
 
 
 @pytest.mark.parametrize("alpha,beta", [(1.0, 1.0), (1.0, 0.0), (-1.0, 1.0), (2.5, -0.5)])
@@ -124,13 +149,14 @@ def test_ir_cg_mixed(kind, r):
     assert np.linalg.norm(xh - xs) / np.linalg.norm(xs) <= 1e-7
 
 
-@pytest.mark.parametrize("tail", [None, 0.0])
+@pytest.mark.parametrize("tail", [None, 0.0, 1e-6])
 @pytest.mark.parametrize("vectors", ["f64", "f32"])
 @pytest.mark.parametrize("kind", ["f32", "bf16"])
 @pytest.mark.parametrize("r", (1, 4))
 def test_mixed_cg_reliable_updates(kind, r, vectors, tail):
     """Reliable-update CG: float iteration, double residual/solution, the
-    true residual below 1e-10 and the solution equal to a dense solve."""
+    true residual below 1e-10 and the solution equal to a dense solve.
+    tail None is the default: a double tail for bfloat16 values only."""
     from spc5_amd.device import DeviceMatrix
     from spc5_amd.solver import lowp, mixed_cg
     rp, col, val = stencil27_small(12)
@@ -140,7 +166,7 @@ def test_mixed_cg_reliable_updates(kind, r, vectors, tail):
     b = np.random.default_rng(6).uniform(-1, 1, n)
     x, st = mixed_cg(A64, Alo, dev(b), rtol=1e-10, vectors=vectors, tail_rtol=tail)
     xh = x.cpu().numpy()
-    if tail == 0.0:
+    if tail == 0.0 or (tail is None and kind == "f32"):
         assert st["tail_iterations"] == 0
     else:
         assert 0 < st["tail_iterations"] < st["iterations"]
