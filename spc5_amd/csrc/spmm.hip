@@ -26,16 +26,29 @@ using namespace spc5;
 namespace spc5 {
 namespace {
 
+// Float and bfloat16 values accumulate their row x column sums in double:
+// the LDS float atomic add (ds_add_f32) of one 64-lane row per record run
+// left the float kernel waiting on LDS for half of its cycles (k = 64 on the
+// 27-point stencil: 13.5 ms with float sums, 4.7 ms with double sums, whose
+// ds_add_f64 is fast).
+#ifndef SPC5_SPMM_FLOAT_ACC_DOUBLE
+#define SPC5_SPMM_FLOAT_ACC_DOUBLE 1
+#endif
+using SpmmFloatAcc = std::conditional<SPC5_SPMM_FLOAT_ACC_DOUBLE != 0, double, float>::type;
+
 constexpr int kSpmmCap = 256;        // values expanded per batch
 constexpr int kSpmmAccBytes = 8192;  // LDS for the row x column sums of a tile
 constexpr int kSpmmMaxKt = 64;       // columns per pass (one lane each)
 
+// One value of a batch: the value, where its X row is (a byte offset from
+// X when X spans less than 4 GiB, else the column) and its tile-local row.
 template <class ACC>
 struct alignas(16) SpmmRec {
     ACC v;
     int col;
     int row;
 };
+static_assert(sizeof(SpmmRec<float>) == 16 && sizeof(SpmmRec<double>) == 16, "16-byte records");
 
 template <class ACC>
 int spmm_max_rows(int kt) {
@@ -50,7 +63,20 @@ size_t spmm_smem_bytes(int maxrows, int kt, int r) {
     return sizeof(SpmmRec<ACC>) * kSpmmCap + sizeof(ACC) * (size_t)maxrows * kt + sizeof(int) * (maxrows / r + 1);
 }
 
-template <class V, class X, class ACC, int R, int VS, int MB, int KC>
+template <bool WIDE, class X>
+__device__ __forceinline__ X spmm_x(const X* __restrict__ xcol, const X* __restrict__ x, int col, int64_t ldx,
+                                    unsigned lane_off) {
+    if constexpr (WIDE) {
+        return xcol[(int64_t)col * ldx];
+    } else {
+        return *reinterpret_cast<const X*>(reinterpret_cast<const char*>(x) + ((unsigned)col + lane_off));
+    }
+}
+
+// WIDE: X spans 4 GiB or more (or rows of 16 MiB or more), so records hold
+// columns and X is addressed with 64-bit arithmetic; otherwise they hold
+// 32-bit byte offsets of the X rows, computed with 24-bit multiplies.
+template <class V, class X, class ACC, int R, int VS, int MB, int KC, bool WIDE>
 __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr, const unsigned char* __restrict__ hdr,
                                                     const V* __restrict__ val, const X* __restrict__ x, int64_t ldx,
                                                     X* __restrict__ y, int64_t ldy, int k, int kt,
@@ -75,6 +101,8 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
     const int jl = lane % KC;
     const bool colok = jl < ktc;
     const X* __restrict__ xcol = x + c0 + (colok ? jl : 0);
+    const unsigned lane_off = (unsigned)(c0 + (colok ? jl : 0)) * (unsigned)sizeof(X);  // byte offset in an X row
+    const unsigned row_bytes = (unsigned)ldx * (unsigned)sizeof(X);
     const int br0 = tile_br[t], br1 = tile_br[t + 1];
     const int nbr = br1 - br0;
     const int b0 = bptr[br0], b1 = bptr[br1];
@@ -114,15 +142,17 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
         const int bn = b + ntake;
         load_header_clamped<R, MB>(hdr, bn + lane, b1 - 1, cs, mw);
         if (lane < ntake) {
-            // records (column, tile-local row) of this block's set bits
+            // records (X row, tile-local row) of this block's set bits
             const int rbase = find_blockrow(rs, nbr, top, jcur - b0) * R;
+            const unsigned cbase = WIDE ? (unsigned)cs_cur : (unsigned)cs_cur * row_bytes;
             int p = incl - cnt;
 #pragma unroll
             for (int w = 0; w < NW; ++w) {
                 unsigned long long m = mw_cur[w];
                 while (m) {
                     const int bit = __builtin_ctzll(m) + w * 64;
-                    rec[p].col = cs_cur + (bit & (VS - 1));
+                    const unsigned c = (unsigned)(bit & (VS - 1));
+                    rec[p].col = (int)(WIDE ? cbase + c : cbase + __umul24(c, row_bytes));
                     rec[p].row = rbase + bit / VS;
                     ++p;
                     m &= m - 1ull;
@@ -145,9 +175,11 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
             Rec rr[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) rr[u] = rec[q + u];
+            // All U record reads are issued before the first use waits.
+            __builtin_amdgcn_sched_barrier(0);
             ACC xv[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) xv[u] = (ACC)xcol[(int64_t)rr[u].col * ldx];
+            for (int u = 0; u < U; ++u) xv[u] = (ACC)spmm_x<WIDE>(xcol, x, rr[u].col, ldx, lane_off);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (rr[u].row != prow) {
@@ -160,7 +192,7 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
         }
         for (; q < qb; ++q) {
             const Rec r1 = rec[q];
-            const ACC xv = (ACC)xcol[(int64_t)r1.col * ldx];
+            const ACC xv = (ACC)spmm_x<WIDE>(xcol, x, r1.col, ldx, lane_off);
             if (r1.row != prow) {
                 if (prow >= 0 && colok) atomicAdd(&acc[prow * kt + jl], run);
                 run = (ACC)0;
@@ -191,7 +223,7 @@ SpmmPlans& spmm_plans() {
     return p;
 }
 
-template <class V, class X, class ACC, int VS, int MB, int KC>
+template <class V, class X, class ACC, int VS, int MB, int KC, bool WIDE>
 int launch_spmm_kc(const spc5_hip_mat& m, const Plan& p, int maxrows, const void* x, int64_t ldx, void* y,
                    int64_t ldy, int k, int kt, double alpha, double beta, hipStream_t s) {
     const dim3 grid((unsigned)p.ntiles, (unsigned)((k + kt - 1) / kt));
@@ -202,7 +234,7 @@ int launch_spmm_kc(const spc5_hip_mat& m, const Plan& p, int maxrows, const void
     const X* xx = reinterpret_cast<const X*>(x);
     X* yy = reinterpret_cast<X*>(y);
 #define SPC5_SPMM_LAUNCH(R)                                                                                   \
-    hipLaunchKernelGGL((k_spmm_tiles<V, X, ACC, R, VS, MB, KC>), grid, dim3(64), shm, s, m.bptr, m.hdr, val, \
+    hipLaunchKernelGGL((k_spmm_tiles<V, X, ACC, R, VS, MB, KC, WIDE>), grid, dim3(64), shm, s, m.bptr, m.hdr, val, \
                        xx, ldx, yy, ldy, k, kt, tb, tv, p.ntiles, m.nrows, maxrows, (ACC)alpha, (ACC)beta)
     switch (m.r) {
     case 1: SPC5_SPMM_LAUNCH(1); break;
@@ -239,15 +271,24 @@ int launch_spmm_t(const spc5_hip_mat& m, const void* x, int64_t ldx, void* y, in
     if (p->ntiles == 0) return SPC5_HIP_OK;
     int kc = 1;  // lanes per record: the smallest power of two holding the chunk's kt columns
     while (kc < kt) kc <<= 1;
+    // 32-bit X offsets when every X row offset plus a column fits, and a
+    // row is under 16 MiB (the 24-bit multiply of the expansion)
+    const uint64_t row_bytes = (uint64_t)ldx * sizeof(X);
+    const bool wide = row_bytes >= (1ull << 24) || (uint64_t)m.ncols * row_bytes > 0xffffffffull;
+#define SPC5_SPMM_KC(KC)                                                                                         \
+    return wide ? launch_spmm_kc<V, X, ACC, VS, MB, KC, true>(m, *p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, \
+                                                               s)                                                 \
+                : launch_spmm_kc<V, X, ACC, VS, MB, KC, false>(m, *p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s)
     switch (kc) {
-    case 64: return launch_spmm_kc<V, X, ACC, VS, MB, 64>(m, *p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s);
-    case 32: return launch_spmm_kc<V, X, ACC, VS, MB, 32>(m, *p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s);
-    case 16: return launch_spmm_kc<V, X, ACC, VS, MB, 16>(m, *p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s);
-    case 8: return launch_spmm_kc<V, X, ACC, VS, MB, 8>(m, *p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s);
-    case 4: return launch_spmm_kc<V, X, ACC, VS, MB, 4>(m, *p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s);
-    case 2: return launch_spmm_kc<V, X, ACC, VS, MB, 2>(m, *p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s);
-    default: return launch_spmm_kc<V, X, ACC, VS, MB, 1>(m, *p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s);
+    case 64: SPC5_SPMM_KC(64);
+    case 32: SPC5_SPMM_KC(32);
+    case 16: SPC5_SPMM_KC(16);
+    case 8: SPC5_SPMM_KC(8);
+    case 4: SPC5_SPMM_KC(4);
+    case 2: SPC5_SPMM_KC(2);
+    default: SPC5_SPMM_KC(1);
     }
+#undef SPC5_SPMM_KC
 }
 
 }  // namespace
@@ -273,10 +314,10 @@ int spc5_hip_spmm(spc5_hip_mat_t h, int k, int kt, double alpha, const void* d_x
     if (h->value_type == SPC5_HIP_F64)
         return launch_spmm_t<double, double, double, 8, 1>(*h, d_x, ldx, d_y, ldy, k, kt, alpha, beta, s);
     if (h->value_type == SPC5_HIP_F32 && h->vs == 16)
-        return launch_spmm_t<float, float, float, 16, 2>(*h, d_x, ldx, d_y, ldy, k, kt, alpha, beta, s);
+        return launch_spmm_t<float, float, SpmmFloatAcc, 16, 2>(*h, d_x, ldx, d_y, ldy, k, kt, alpha, beta, s);
     if (h->value_type == SPC5_HIP_F32)
-        return launch_spmm_t<float, float, float, 8, 1>(*h, d_x, ldx, d_y, ldy, k, kt, alpha, beta, s);
-    return launch_spmm_t<unsigned short, float, float, 8, 1>(*h, d_x, ldx, d_y, ldy, k, kt, alpha, beta, s);
+        return launch_spmm_t<float, float, SpmmFloatAcc, 8, 1>(*h, d_x, ldx, d_y, ldy, k, kt, alpha, beta, s);
+    return launch_spmm_t<unsigned short, float, SpmmFloatAcc, 8, 1>(*h, d_x, ldx, d_y, ldy, k, kt, alpha, beta, s);
 }
 
 }  // extern "C"
