@@ -6,8 +6,10 @@
 // per entry, built once at batch creation).  A persistent grid of
 // `waves` waves walks the list with a static stride; each tile runs the
 // same tile routine as the single-matrix kernel, with the matrix's own r
-// (chosen per matrix, e.g. by the autotuner) selected at run time.  The
-// launch is captured once into a hipGraph; every product replays it.
+// (chosen per matrix, e.g. by the autotuner) selected at run time.  When
+// every matrix has the same r, a kernel holding only that r's body runs
+// instead (the four bodies together need far more registers than one).
+// The launch is captured once into a hipGraph; every product replays it.
 #include "spc5_mat.hpp"
 #include "spmv_tile.hpp"
 
@@ -36,6 +38,7 @@ struct spc5_hip_batch {
     int nmats = 0;
     int64_t ntiles = 0;
     int waves = 0;
+    int only_r = 0;         // the r shared by every matrix, 0 if they differ
     DeviceBuffer entries;   // BatchEntry[nmats]
     DeviceBuffer tile_mat;  // int32[ntiles]: matrix of each tile
     DeviceBuffer tile_loc;  // int32[ntiles]: tile index inside its matrix
@@ -63,8 +66,16 @@ union BatchSmem {
 #define SPC5_BATCH_TILE(RR, SMEM) \
     spmv_tile<V, X, ACC, RR, VS, MB>(SMEM, m.bptr, m.hdr, val, x, y, br0, br1, v0, v1, m.nrows, m.ncols, ep)
 
-template <class V, class X, class ACC, int VS, int MB>
-__global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_batch(const BatchEntry* __restrict__ mats,
+// Waves per SIMD of the batch kernel: with every r body (ONLY_R = 0) the
+// most that compile without spills; with one r, the tile kernel's rule.
+constexpr int batch_waves_per_eu(int only_r, bool wide_x, int value_bytes) {
+    // the persistent loop's own state costs one wave (two with 8-byte values)
+    return only_r == 0 ? 5 : tile_waves_per_eu(only_r, false, wide_x) - (value_bytes == 8 ? 2 : 1);
+}
+
+template <class V, class X, class ACC, int VS, int MB, int ONLY_R>
+__global__ __launch_bounds__(kWavesPerGroup * 64, batch_waves_per_eu(ONLY_R, sizeof(X) > sizeof(ACC), sizeof(V))) void
+k_spmv_batch(const BatchEntry* __restrict__ mats,
                                                                      const int* __restrict__ tile_mat,
                                                                      const int* __restrict__ tile_loc,
                                                                      int64_t ntiles) {
@@ -72,14 +83,18 @@ __global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_batch(const BatchE
     const int w = threadIdx.x >> 6;
     const int64_t stride = (int64_t)gridDim.x * kWavesPerGroup;
     const Epilogue ep;
-    for (int64_t t = (int64_t)blockIdx.x * kWavesPerGroup + w; t < ntiles; t += stride) {
+    // The tile index is the same on every lane (it comes from the wave index,
+    // which the compiler cannot tell is uniform): read it as a scalar, so the
+    // matrix descriptor is loaded into scalar registers.
+    const int64_t t0 = (int64_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWavesPerGroup + w));
+    for (int64_t t = t0; t < ntiles; t += stride) {
         const BatchEntry& m = mats[tile_mat[t]];
         const int lt = tile_loc[t];
         const int br0 = m.tile_br[lt], br1 = m.tile_br[lt + 1], v0 = m.tile_val[lt], v1 = m.tile_val[lt + 1];
         const V* val = reinterpret_cast<const V*>(m.val);
         const X* x = reinterpret_cast<const X*>(m.x);
         X* y = reinterpret_cast<X*>(m.y);
-        switch (m.r) {
+        switch (ONLY_R ? ONLY_R : m.r) {
         case 1:
             SPC5_BATCH_TILE(1, sm[w].r1);
             break;
@@ -97,12 +112,29 @@ __global__ __launch_bounds__(kWavesPerGroup * 64) void k_spmv_batch(const BatchE
     }
 }
 
-template <class V, class X, class ACC, int VS, int MB>
-void launch_batch(const spc5_hip_batch& b, hipStream_t s) {
+template <class V, class X, class ACC, int VS, int MB, int ONLY_R>
+void launch_batch_r(const spc5_hip_batch& b, hipStream_t s) {
     const unsigned groups = (unsigned)((b.waves + kWavesPerGroup - 1) / kWavesPerGroup);
-    hipLaunchKernelGGL((k_spmv_batch<V, X, ACC, VS, MB>), dim3(groups), dim3(kWavesPerGroup * 64), 0, s,
+    hipLaunchKernelGGL((k_spmv_batch<V, X, ACC, VS, MB, ONLY_R>), dim3(groups), dim3(kWavesPerGroup * 64), 0, s,
                        b.entries.as<const BatchEntry>(), b.tile_mat.as<const int>(), b.tile_loc.as<const int>(),
                        b.ntiles);
+}
+
+template <class V, class X, class ACC, int VS, int MB>
+void launch_batch(const spc5_hip_batch& b, hipStream_t s) {
+    switch (b.only_r) {
+    case 1: launch_batch_r<V, X, ACC, VS, MB, 1>(b, s); break;
+    case 2: launch_batch_r<V, X, ACC, VS, MB, 2>(b, s); break;
+    case 4: launch_batch_r<V, X, ACC, VS, MB, 4>(b, s); break;
+    case 8: launch_batch_r<V, X, ACC, VS, MB, 8>(b, s); break;
+    default: launch_batch_r<V, X, ACC, VS, MB, 0>(b, s); break;
+    }
+}
+
+// Waves per SIMD the batch's kernel runs at (sizes the persistent grid).
+int batch_kernel_waves(int value_type, int only_r) {
+    const bool wide_x = false;  // batches use the matrices' own vector type
+    return batch_waves_per_eu(only_r, wide_x, value_type == SPC5_HIP_F64 ? 8 : value_type == SPC5_HIP_F32 ? 4 : 2);
 }
 
 int launch_batch_any(const spc5_hip_batch& b, hipStream_t s) {
@@ -161,7 +193,10 @@ int spc5_hip_batch_create(int nmats, const spc5_hip_mat_t* mats, const void* con
     int dev = 0, ncu = 256;
     SPC5_HIP_TRY(hipGetDevice(&dev));
     SPC5_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    int64_t wv = waves > 0 ? waves : (int64_t)ncu * 16;
+    b->only_r = mats[0]->r;
+    for (int i = 1; i < nmats; ++i)
+        if (mats[i]->r != b->only_r) b->only_r = 0;
+    int64_t wv = waves > 0 ? waves : (int64_t)ncu * 4 * batch_kernel_waves(b->value_type, b->only_r);
     if (wv > b->ntiles) wv = std::max<int64_t>(b->ntiles, 1);
     b->waves = (int)wv;
     SPC5_HIP_TRY(b->entries.alloc(sizeof(BatchEntry) * (size_t)nmats));

@@ -43,17 +43,6 @@ __device__ __forceinline__ int64_t xcd_tile(int wave) {
     return gx * kWavesPerGroup + wave;
 }
 
-// Waves per SIMD each tile kernel is compiled for (a register budget of
-// 512 / waves VGPRs): the most that compile without spills.  0 leaves the
-// choice to the compiler.
-#ifndef SPC5_WAVES_PER_EU
-#define SPC5_WAVES_PER_EU 8
-#endif
-constexpr int tile_waves_per_eu(int r, bool sorted, bool wide_x) {
-    // wide_x: double x with float sums (twice the registers per x value)
-    return SPC5_WAVES_PER_EU == 0 ? 1
-                                  : (sorted ? (r == 2 ? 6 : 5) : (r == 8 ? 7 : SPC5_WAVES_PER_EU)) - (wide_x ? 1 : 0);
-}
 
 template <class V, class X, class ACC, int R, int VS, int MB, bool SORT>
 __global__ __launch_bounds__(kWavesPerGroup * 64, tile_waves_per_eu(R, SORT, sizeof(X) > sizeof(ACC))) void

@@ -45,6 +45,18 @@ struct RowSort {
     static constexpr int kWords = (R + 2) / 3;  // packed words of three 10-bit row counts
 };
 
+// Waves per SIMD each tile kernel is compiled for (a register budget of
+// 512 / waves VGPRs): the most that compile without spills.  0 leaves the
+// choice to the compiler.
+#ifndef SPC5_WAVES_PER_EU
+#define SPC5_WAVES_PER_EU 8
+#endif
+constexpr int tile_waves_per_eu(int r, bool sorted, bool wide_x) {
+    // wide_x: double x with float sums (twice the registers per x value)
+    return SPC5_WAVES_PER_EU == 0 ? 1
+                                  : (sorted ? (r == 2 ? 6 : 5) : (r == 8 ? 7 : SPC5_WAVES_PER_EU)) - (wide_x ? 1 : 0);
+}
+
 constexpr int kNoRow = kMaxRows;  // row byte of the positions past the batch's end
 
 

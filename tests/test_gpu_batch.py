@@ -18,7 +18,10 @@ def dev(a):
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 1e-5)])
-def test_batch_uneven_matrices(dtype, tol):
+@pytest.mark.parametrize("same_r", [0, 1, 2, 4, 8])
+def test_batch_uneven_matrices(dtype, tol, same_r):
+    """same_r 0: r cycles through 1, 2, 4, 8 (the kernel with every r);
+    otherwise every matrix has that r (the kernel with only its code)."""
     from spc5_amd.batch import Batch
     from spc5_amd.device import DeviceMatrix
     rng = np.random.default_rng(0)
@@ -30,7 +33,7 @@ def test_batch_uneven_matrices(dtype, tol):
             rp, col, v = random_csr(n, nc, 0.03, seed=i, dtype=dtype)
         else:
             rp, col, v = clustered_csr(n, nc, seed=i, dtype=dtype, empty_frac=0.1)
-        r = (1, 2, 4, 8)[i % 4]
+        r = same_r if same_r else (1, 2, 4, 8)[i % 4]
         m = DeviceMatrix.from_csr(dev(rp), dev(col), dev(v), r, ncols=nc)
         x = rng.uniform(-1, 1, nc).astype(dtype)
         y0 = rng.uniform(-1, 1, n).astype(dtype)
