@@ -54,3 +54,34 @@ def test_spmm_strided_and_beta0():
     A.spmm(X, Y, alpha=2.0, beta=0.0)      # beta = 0: Y is not read
     ref = 2.0 * dense_of(rp, col, val, 300) @ X.cpu().numpy()
     assert np.abs(Y.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max() * 10
+
+
+def test_spmm_wide_x_rows():
+    """X with a leading dimension that puts it beyond 4 GiB: the SpMM body
+    with 64-bit X addressing; reference from the CSR."""
+    import numpy as np
+    import torch
+    from spc5_amd.device import DeviceMatrix
+    rng = np.random.default_rng(5)
+    n, ncols, k, ldx = 200, 70_000, 8, 8200  # ncols * ldx * 8 > 4 GiB
+    sets = [np.unique(np.concatenate([rng.integers(0, ncols, 5), [ncols - 1 - i % 7]])) for i in range(n)]
+    rowptr = np.zeros(n + 1, np.int32)
+    for i, c in enumerate(sets):
+        rowptr[i + 1] = rowptr[i] + len(c)
+    col = np.concatenate(sets).astype(np.int32)
+    v = rng.uniform(-1, 1, len(col))
+    X = torch.zeros((ncols, ldx), dtype=torch.float64, device="cuda")
+    used = np.unique(col)
+    xu = rng.uniform(-1, 1, (len(used), k))
+    X[torch.from_numpy(used.astype(np.int64)).cuda(), :k] = torch.from_numpy(xu).cuda()
+    xmap = {int(c): xu[j] for j, c in enumerate(used)}
+    A = DeviceMatrix.from_csr(torch.from_numpy(rowptr).cuda(), torch.from_numpy(col).cuda(),
+                              torch.from_numpy(v).cuda(), 2, ncols=ncols)
+    Y = torch.zeros((n, k), dtype=torch.float64, device="cuda")
+    A.spmm(X[:, :k], Y, beta=0.0)  # a row-major view with leading dimension ldx
+    torch.cuda.synchronize()
+    Yref = np.array([sum(v[j] * xmap[int(col[j])] for j in range(rowptr[i], rowptr[i + 1])) for i in range(n)])
+    scale = np.abs(v).max() * np.diff(rowptr).max()
+    assert np.abs(Y.cpu().numpy() - Yref).max() <= 1e-12 * scale
+    del X
+    torch.cuda.empty_cache()

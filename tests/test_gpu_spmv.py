@@ -321,3 +321,36 @@ def test_gpu_stencil27(tag, r, row_sort):
     x, y0, bptr, hdr, vout, yref = _oracle_case(rp, col, v, r, 7, n)
     y = gpu_spmv_from_arrays(r, n, n, bptr, hdr, vout, x, y0, row_sort=row_sort)
     assert normwise_error(y, yref[:n], rp, v, x[:n], y0[:n]) <= TOL[tag]
+
+
+@pytest.mark.parametrize("r", (1, 4))
+@pytest.mark.parametrize("row_sort", [False, True])
+def test_gpu_x_beyond_4gib(r, row_sort):
+    """x spanning more than 4 GiB (ncols * 8 bytes): the kernel body with
+    64-bit x addressing.  A few rows with columns spread up to the end of x;
+    reference from the CSR and the x entries it touches."""
+    from spc5_amd.device import DeviceMatrix
+    ncols = (1 << 29) + 4099  # 4 GiB + 32 KiB of doubles
+    rng = np.random.default_rng(11)
+    n = 300
+    sets = [np.unique(np.concatenate([rng.integers(0, ncols, 6), rng.integers(ncols - 40, ncols, 3),
+                                      [i % 50]])) for i in range(n)]
+    rowptr = np.zeros(n + 1, np.int32)
+    for i, c in enumerate(sets):
+        rowptr[i + 1] = rowptr[i] + len(c)
+    col = np.concatenate(sets).astype(np.int32)
+    v = rng.uniform(-1, 1, len(col))
+    x = torch.zeros(ncols, dtype=torch.float64, device="cuda")
+    used = np.unique(col)
+    xu = rng.uniform(-1, 1, len(used))
+    x[dev(used.astype(np.int64))] = dev(xu)
+    xmap = dict(zip(used.tolist(), xu.tolist()))
+    m = DeviceMatrix.from_csr(dev(rowptr), dev(col), dev(v), r, ncols=ncols)
+    m.row_sort = row_sort
+    y = torch.zeros(n, dtype=torch.float64, device="cuda")
+    m.spmv(x, y)
+    yref = np.array([sum(v[k] * xmap[int(col[k])] for k in range(rowptr[i], rowptr[i + 1])) for i in range(n)])
+    scale = np.abs(v).max() * np.diff(rowptr).max()
+    assert np.abs(host(y) - yref).max() <= 1e-12 * scale
+    del x
+    torch.cuda.empty_cache()
