@@ -143,6 +143,25 @@ int spc5_hip_mat_set_option(spc5_hip_mat_t h, int option, int64_t value);
  * OpenMP driver SPC5_<r>rVc_Spmv_omp (src/spc5.hpp:557-599 etc.). */
 int spc5_hip_spmv(spc5_hip_mat_t h, const void* d_x, void* d_y, void* stream);
 
+/* COO (device) -> CSR (device): the reference's COO_unsorted_to_CSR
+ * (src/spc5.hpp:106-114; sorted = 0) and COO_sorted_to_CSR (77-104;
+ * sorted = 1, order checked).  Entries are ordered by (row, column) with a
+ * stable radix sort; duplicates are kept in input order.  value_bytes is 8
+ * (double) or 4 (float).  d_rowptr gets nrows+1 entries, d_col / d_val nnz.
+ * Indices out of range or an unsorted input with sorted = 1 are errors. */
+int spc5_hip_coo_to_csr(int value_bytes, int64_t nrows, int64_t ncols, int64_t nnz, int sorted, const int* d_i,
+                        const int* d_j, const void* d_v, int* d_rowptr, int* d_col, void* d_val, void* stream);
+
+/* MatrixMarket coordinate files (real, integer or pattern; general,
+ * symmetric or skew-symmetric), read as examples/load_mm_and_compare.cpp
+ * loadMM (287-428) reads them: 0-based indices, pattern values 1, the
+ * mirrored entry of every off-diagonal symmetric (negated for
+ * skew-symmetric) entry right after it.  info gives the dimensions and the
+ * entry count after mirroring; read fills host arrays of `capacity` entries
+ * (the sort by (row, column) is spc5_hip_coo_to_csr). */
+int spc5_hip_mm_info(const char* path, int64_t* nrows, int64_t* ncols, int64_t* nentries);
+int spc5_hip_mm_read(const char* path, int64_t capacity, int* i, int* j, double* v, int64_t* nread);
+
 /* GPU converter, CSR (device) -> beta(r,VS), in two calls.
  * count: writes the block-row pointers into d_rowsSize_out[0..ceil(N/r)]
  *   (count pass + exclusive scan) and returns the block count in
