@@ -128,6 +128,16 @@ int spc5_hip_mat_destroy(spc5_hip_mat_t h);
 int spc5_hip_mat_info(spc5_hip_mat_t h, int64_t* ntiles, int64_t* nblockrows, int64_t* nblocks,
                       int64_t* nnz);
 
+/* Serialisation of a matrix in the beta(r,VS) layout (no counterpart in
+ * the reference; amortises conversion across runs).  save copies the
+ * handle's arrays to a file; load reads one into a handle that owns its
+ * device copies (freed by spc5_hip_mat_destroy) and builds its plan.  File:
+ * "SPC5BRVC", uint32 version 1, int32 value type / VS / mask bytes / r / 0,
+ * int64 nrows / ncols / nnz / nblocks, then rowsSize[ceil(nrows/r)+1]
+ * (int32), the headers and the values, little-endian. */
+int spc5_hip_mat_save(spc5_hip_mat_t h, const char* path);
+int spc5_hip_mat_load(const char* path, spc5_hip_mat_t* out);
+
 /* Per-handle kernel options (spc5_hip_mat_set_option).
  * ROW_SORT (0 or 1, default 0; used for r = 2 and 4): the products of every batch of
  * blocks are summed in row order instead of block order.  Pays off where

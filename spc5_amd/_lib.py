@@ -39,6 +39,8 @@ SIGNATURES = {
     "spc5_hip_mat_info": (_I, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64)]),
     "spc5_hip_mat_set_option": (_I, [_P, _I, _I64]),
     "spc5_hip_coo_to_csr": (_I, [_I, _I64, _I64, _I64, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "spc5_hip_mat_save": (_I, [_P, C.c_char_p]),
+    "spc5_hip_mat_load": (_I, [C.c_char_p, C.POINTER(_P)]),
     "spc5_hip_mm_info": (_I, [C.c_char_p, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64)]),
     "spc5_hip_mm_read": (_I, [C.c_char_p, _I64, _P, _P, _P, C.POINTER(_I64)]),
     "spc5_hip_spmv": (_I, [_P, _P, _P, _P]),

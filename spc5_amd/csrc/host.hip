@@ -5,6 +5,8 @@
 // any HIP header.
 #include "spc5_mat.hpp"
 
+#include <cstdio>
+#include <cstring>
 #include <vector>
 
 using namespace spc5;
@@ -131,6 +133,84 @@ int spc5_hip_spmv_host(spc5_hip_mat_t h, const void* x, void* y) {
     if (rc) return rc;
     SPC5_HIP_TRY(hipMemcpy(y, dy.p, es * (size_t)h->nrows, hipMemcpyDeviceToHost));
     return SPC5_HIP_OK;
+}
+
+// ---- serialisation ----------------------------------------------------------
+// File layout, little-endian:
+//   char   magic[8] = "SPC5BRVC"; uint32 version = 1;
+//   int32  value_type, vs, mask_bytes, r, reserved (0);
+//   int64  nrows, ncols, nnz, nblocks;
+//   int32  rowsSize[ceil(nrows/r) + 1]     (block-row pointers, first 0)
+//   uint8  headers[nblocks * (4 + r * mask_bytes)]
+//   values[nnz]                            (8, 4 or 2 bytes each)
+
+int spc5_hip_mat_save(spc5_hip_mat_t h, const char* path) {
+    SPC5_CHECK_ARG(h != nullptr && path != nullptr, "null handle or path");
+    const int64_t nbr = h->nbr;
+    std::vector<int> bp((size_t)nbr + 1);
+    std::vector<unsigned char> hdr((size_t)h->nblocks * (size_t)header_bytes(h->r, h->mb));
+    std::vector<unsigned char> val((size_t)h->nnz * value_bytes(h->value_type));
+    SPC5_HIP_TRY(hipDeviceSynchronize());
+    SPC5_HIP_TRY(hipMemcpy(bp.data(), h->bptr, sizeof(int) * bp.size(), hipMemcpyDeviceToHost));
+    if (!hdr.empty()) SPC5_HIP_TRY(hipMemcpy(hdr.data(), h->hdr, hdr.size(), hipMemcpyDeviceToHost));
+    if (!val.empty()) SPC5_HIP_TRY(hipMemcpy(val.data(), h->values, val.size(), hipMemcpyDeviceToHost));
+    FILE* f = std::fopen(path, "wb");
+    if (!f) {
+        set_error(std::string("cannot write ") + path);
+        return SPC5_HIP_ERR_INVALID;
+    }
+    const uint32_t version = 1;
+    const int32_t meta[5] = {h->value_type, h->vs, h->mb, h->r, 0};
+    const int64_t dims[4] = {h->nrows, h->ncols, h->nnz, h->nblocks};
+    bool ok = std::fwrite("SPC5BRVC", 1, 8, f) == 8 && std::fwrite(&version, 4, 1, f) == 1 &&
+              std::fwrite(meta, 4, 5, f) == 5 && std::fwrite(dims, 8, 4, f) == 4 &&
+              std::fwrite(bp.data(), 4, bp.size(), f) == bp.size() &&
+              std::fwrite(hdr.data(), 1, hdr.size(), f) == hdr.size() &&
+              std::fwrite(val.data(), 1, val.size(), f) == val.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) {
+        set_error(std::string("short write to ") + path);
+        return SPC5_HIP_ERR_INVALID;
+    }
+    return SPC5_HIP_OK;
+}
+
+int spc5_hip_mat_load(const char* path, spc5_hip_mat_t* out) {
+    SPC5_CHECK_ARG(path != nullptr && out != nullptr, "null path or out");
+    *out = nullptr;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) {
+        set_error(std::string("cannot read ") + path);
+        return SPC5_HIP_ERR_INVALID;
+    }
+    char magic[8];
+    uint32_t version = 0;
+    int32_t meta[5];
+    int64_t dims[4];
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "SPC5BRVC", 8) == 0 &&
+              std::fread(&version, 4, 1, f) == 1 && version == 1 && std::fread(meta, 4, 5, f) == 5 &&
+              std::fread(dims, 8, 4, f) == 4;
+    const int vt = meta[0], vs = meta[1], mb = meta[2], r = meta[3];
+    ok = ok && (r == 1 || r == 2 || r == 4 || r == 8) && valid_format(vt, vs, mb) && dims[0] >= 0 && dims[1] >= 0 &&
+         dims[2] >= 0 && dims[3] >= 0 && dims[2] <= INT32_MAX && dims[3] <= INT32_MAX && dims[0] <= INT32_MAX;
+    if (!ok) {
+        std::fclose(f);
+        set_error(std::string("not a beta(r,VS) matrix file: ") + path);
+        return SPC5_HIP_ERR_INVALID;
+    }
+    const int64_t nbr = (dims[0] + r - 1) / r;
+    std::vector<int> bp((size_t)nbr + 1);
+    std::vector<unsigned char> hdr((size_t)dims[3] * (size_t)header_bytes(r, mb));
+    std::vector<unsigned char> val((size_t)dims[2] * value_bytes(vt));
+    ok = std::fread(bp.data(), 4, bp.size(), f) == bp.size() && std::fread(hdr.data(), 1, hdr.size(), f) == hdr.size() &&
+         std::fread(val.data(), 1, val.size(), f) == val.size();
+    std::fclose(f);
+    if (!ok) {
+        set_error(std::string("truncated matrix file: ") + path);
+        return SPC5_HIP_ERR_INVALID;
+    }
+    return spc5_hip_mat_create_from_host(vt, vs, r, dims[0], dims[1], dims[2], dims[3], bp.data(), hdr.data(),
+                                         val.data(), out);
 }
 
 }  // extern "C"

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
 import torch
 
 from ._lib import SPC5_HIP_BF16, SPC5_HIP_F32, SPC5_HIP_F64, SPC5_HIP_OPT_ROW_SORT, Spc5Error, check, lib
@@ -119,6 +120,42 @@ class DeviceMatrix:
         vs = default_vs(values.dtype) if vs is None else vs
         return cls(r=r, vs=vs, nrows=n, ncols=n if ncols is None else ncols, nnz=values.numel(), nblocks=nblocks,
                    rows_size=rows_size, headers=hdr, values=vout, stream=stream)
+
+    def save(self, path: str):
+        """Write the matrix (beta(r,VS) arrays) to `path`; see load."""
+        check(lib().spc5_hip_mat_save(self._h, str(path).encode()), "spc5_hip_mat_save")
+
+    @classmethod
+    def load(cls, path: str, stream=None):
+        """Read a matrix written by save (or spc5_hip_mat_save) into HBM and
+        build its plan.  The layout is the one include/spc5_hip.h gives for
+        spc5_hip_mat_save."""
+        with open(path, "rb") as f:
+            blob = f.read()
+        if blob[:8] != b"SPC5BRVC" or int(np.frombuffer(blob, np.uint32, 1, 8)[0]) != 1:
+            raise Spc5Error(f"not a beta(r,VS) matrix file: {path}")
+        vt, vs, mb, r, _ = (int(a) for a in np.frombuffer(blob, np.int32, 5, 12))
+        nrows, ncols, nnz, nblocks = (int(a) for a in np.frombuffer(blob, np.int64, 4, 32))
+        if vt not in _VT.values() or r not in FORMAT_NAMES or mb != mask_bytes(vs):
+            raise Spc5Error(f"unsupported matrix file: {path}")
+        nbr = (nrows + r - 1) // r
+        off = 64
+        rs = np.frombuffer(blob, np.int32, nbr + 1, off)
+        off += 4 * (nbr + 1)
+        hb = nblocks * (4 + r * mb)
+        hdr = np.frombuffer(blob, np.uint8, hb, off)
+        off += hb
+        vdt = {SPC5_HIP_F64: np.float64, SPC5_HIP_F32: np.float32, SPC5_HIP_BF16: np.int16}[vt]
+        vals = np.frombuffer(blob, vdt, nnz, off)
+        if off + vals.nbytes != len(blob):
+            raise Spc5Error(f"matrix file size does not match its header: {path}")
+        dev = lambda a: torch.from_numpy(np.array(a)).cuda()
+        v = dev(vals)
+        if vt == SPC5_HIP_BF16:
+            v = v.view(torch.bfloat16)
+        return cls(r=r, vs=vs, nrows=nrows, ncols=ncols, nnz=nnz, nblocks=nblocks, rows_size=dev(rs),
+                   headers=dev(hdr) if hb else torch.zeros(1, dtype=torch.uint8, device="cuda"), values=v,
+                   stream=stream)
 
     def values_view(self, values):
         """Same block structure and plan, other values (e.g. float32/bfloat16

@@ -354,3 +354,42 @@ def test_gpu_x_beyond_4gib(r, row_sort):
     assert np.abs(host(y) - yref).max() <= 1e-12 * scale
     del x
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("r", RS)
+@pytest.mark.parametrize("kind", ["f64", "f32vs16", "bf16view"])
+def test_save_load_round_trip(tmp_path, r, kind):
+    """A matrix written by spc5_hip_mat_save reads back (Python reader and
+    the C-ABI loader) with identical arrays and an identical product."""
+    import ctypes as C
+    from spc5_amd._lib import check, lib
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import lowp
+    rp, col, v = clustered_csr(1500, 2000, r, empty_frac=0.1)
+    m = DeviceMatrix.from_csr(dev(rp), dev(col), dev(v if kind != "f32vs16" else v.astype(np.float32)), r,
+                              ncols=2000)
+    if kind == "bf16view":
+        m = lowp(m, "bf16")
+    p = str(tmp_path / "m.spc5")
+    m.save(p)
+    m2 = DeviceMatrix.load(p)
+    assert (m2.r, m2.vs, m2.nrows, m2.ncols, m2.nnz, m2.nblocks) == (m.r, m.vs, m.nrows, m.ncols, m.nnz, m.nblocks)
+    nbr = (m.nrows + r - 1) // r
+    hb = m.nblocks * m.header_bytes()
+    assert torch.equal(m.rows_size[: nbr + 1].cpu() - m.rows_size[0].cpu(), m2.rows_size[: nbr + 1].cpu())
+    assert torch.equal(m.headers[:hb].cpu(), m2.headers[:hb].cpu())
+    bits = torch.int16 if m.values.dtype == torch.bfloat16 else m.values.dtype
+    assert torch.equal(m.values[: m.nnz].cpu().view(bits), m2.values[: m.nnz].cpu().view(bits))
+    x = torch.from_numpy(np.random.default_rng(r).uniform(-1, 1, 2000)).cuda().to(m.vec_dtype)
+    y1 = torch.zeros(1500, dtype=m.vec_dtype, device="cuda")
+    y2 = torch.zeros_like(y1)
+    m.spmv(x, y1)
+    m2.spmv(x, y2)
+    assert torch.equal(y1, y2)
+    h = C.c_void_p(0)  # the C-ABI loader
+    check(lib().spc5_hip_mat_load(p.encode(), C.byref(h)), "spc5_hip_mat_load")
+    y3 = torch.zeros_like(y1)
+    check(lib().spc5_hip_spmv(h, x.data_ptr(), y3.data_ptr(), None), "spc5_hip_spmv")
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y3)
+    check(lib().spc5_hip_mat_destroy(h), "destroy")
