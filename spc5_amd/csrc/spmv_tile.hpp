@@ -6,10 +6,12 @@
 //      ahead; popcounts and a DPP wave scan give every block's value offset
 //      inside the batch; blocks are taken while the batch holds at most
 //      expand_cap<V>() values;
-//   2. the batch's values are loaded at once, expand_cap<V>()/64 coalesced loads
-//      per lane, straight from the value stream (no zero padding), through
-//      a buffer resource based at the batch's first value (constant lane
-//      offsets; the range check returns 0 past the tile's values);
+//   2. the batch's values are loaded at once, expand_cap<V>()/64 coalesced
+//      loads per lane at most, straight from the value stream (no zero
+//      padding), through a buffer resource based at the batch's first value
+//      and as long as the batch (constant lane offsets; the range check
+//      returns 0 past the batch's end; 64-value chunks wholly past it are
+//      not issued);
 //   3. the next batch's headers are requested;
 //   4. the block-row of every block of the batch comes from one pass over
 //      the tile's block-row pointers in LDS (the block-rows that start in
@@ -59,14 +61,12 @@ constexpr int tile_waves_per_eu(int r, bool sorted, bool wide_x) {
 
 constexpr int kNoRow = kMaxRows;  // row byte of the positions past the batch's end
 
-
 // LDS of one wave's tile: CAP values per batch.
 template <class ACC, int R, int CAP, bool kSort = false>
 struct TileSmem {
     ACC acc[kMaxRows];  // row sums of the tile
-    // Columns in value order; every column is read before the first
-    // product is stored over them.
-    // (x byte offsets instead of columns when x spans less than 4 GiB).
+    // Columns in value order (x byte offsets when x spans less than 4 GiB);
+    // every column is read before the first product is stored over them.
     union alignas(16) {
         int col[CAP];
         int2 cp[RowSort<R, kSort>::kOn ? CAP : 1];  // row-sorted: column and row-sorted position
