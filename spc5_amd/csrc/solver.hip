@@ -63,22 +63,44 @@ __device__ __forceinline__ double block_sum(double v, double* s_part) {
 }
 
 // out = sum of part[0..n) in a fixed order (one workgroup of kSumThreads):
-// thread t adds part[t], part[t + kSumThreads], ... into kSumLanes
-// independent accumulators (so its loads are all in flight together), then
-// the accumulators, the wave and the waves are combined in a fixed order.
+// thread t loads part[t], part[t + kSumThreads], ... kSumLoads at a time
+// (all in flight together: one pass covers 32K partials, one per tile of a
+// 4M-row matrix) and adds load k into accumulator k % kSumLanes; then the
+// accumulators, the wave and the waves are combined in a fixed order.
 constexpr int kSumThreads = 1024;
 constexpr int kSumLanes = 8;
+constexpr int kSumLoads = 32;
 __global__ __launch_bounds__(kSumThreads) void k_reduce_sum(const double* __restrict__ part, int64_t n,
                                                             double* __restrict__ out) {
     __shared__ double s_part[kSumThreads / 64];
     double acc[kSumLanes];
 #pragma unroll
     for (int k = 0; k < kSumLanes; ++k) acc[k] = 0.0;
-    for (int64_t i0 = 0; i0 < n; i0 += (int64_t)kSumThreads * kSumLanes) {
+    if (n <= 0) {
+        if (threadIdx.x == 0) *out = 0.0;
+        return;
+    }
+    // partial counts (one per tile or per vector workgroup) are far below
+    // 2^31, so the offsets from the pass base are 32-bit
+    for (int64_t i0 = 0; i0 < n; i0 += (int64_t)kSumThreads * kSumLoads) {
+        const double* base = part + i0;
+        const unsigned m = (unsigned)(n - i0 < (int64_t)kSumThreads * kSumLoads ? n - i0 : kSumThreads * kSumLoads);
+        // Every load is unconditional (index clamped) and the zero for past
+        // the end is selected afterwards: a load under a lane condition is
+        // merged with its default right after it, which waits for it, and
+        // the loads would run one after another.
+        double v[kSumLoads];
 #pragma unroll
-        for (int k = 0; k < kSumLanes; ++k) {
-            const int64_t i = i0 + (int64_t)k * kSumThreads + threadIdx.x;
-            acc[k] += i < n ? part[i] : 0.0;
+        for (int k = 0; k < kSumLoads; ++k) {
+            const unsigned i = (unsigned)k * kSumThreads + threadIdx.x;
+            // a 32-bit byte offset from a uniform base: one address register
+            const unsigned off = (i < m ? i : m - 1) * (unsigned)sizeof(double);
+            v[k] = *(const double*)((const char*)base + off);
+        }
+#pragma unroll
+        for (int k = 0; k < kSumLoads; ++k) {
+            const unsigned i = (unsigned)k * kSumThreads + threadIdx.x;
+            acc[k % kSumLanes] += i < m ? v[k] : 0.0;
         }
     }
     double p = 0.0;
@@ -151,13 +173,46 @@ __global__ __launch_bounds__(kVecThreads) void k_cg_r(X* __restrict__ r, const X
     if (threadIdx.x == 0) part[blockIdx.x] = p;
 }
 
-// K3: x += alpha p; p = r + beta p
+// sum of part[0..m) (m <= kMaxVecGroups) in a fixed order, the same value in
+// every thread of every workgroup that calls it: thread t adds part[t],
+// part[t + kVecThreads], ...; the xor butterfly leaves identical sums in all
+// lanes (each step adds the same two values in either order), and every
+// thread adds the wave sums in wave order.  Replaces a separate reduction
+// launch: the partials stay in L2 and each workgroup reads them itself.
+__device__ __forceinline__ double sum_partials_all(const double* __restrict__ part, unsigned m,
+                                                   double* __restrict__ s_part) {
+    constexpr int kLoads = kMaxVecGroups / kVecThreads;
+    double v[kLoads];
+#pragma unroll
+    for (int k = 0; k < kLoads; ++k) {
+        const unsigned i = (unsigned)k * kVecThreads + threadIdx.x;
+        const unsigned off = (i < m ? i : m - 1) * (unsigned)sizeof(double);
+        v[k] = *(const double*)((const char*)part + off);
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kLoads; ++k) acc += (unsigned)k * kVecThreads + threadIdx.x < m ? v[k] : 0.0;
+    acc = wave_sum_f64(acc);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kVecThreads / 64; ++w) t += s_part[w];
+    return t;
+}
+
+// K3: rr' = sum of the m partials of r.r left by K2 (workgroup 0 stores it in
+// sc->rr[par ^ 1]); x += alpha p; p = r + beta p
 template <class X>
 __global__ __launch_bounds__(kVecThreads) void k_cg_xp(X* __restrict__ x, X* __restrict__ p, const X* __restrict__ r,
-                                                       int64_t n, const CgScalars* __restrict__ sc, int par) {
+                                                       int64_t n, CgScalars* __restrict__ sc, int par,
+                                                       const double* __restrict__ part, unsigned m) {
+    __shared__ double s_part[kVecThreads / 64];
     const double rr = sc->rr[par];
+    const double rr_new = sum_partials_all(part, m, s_part);
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->rr[par ^ 1] = rr_new;
     const X alpha = (X)(rr / sc->pq);
-    const X beta = (X)(sc->rr[par ^ 1] / rr);
+    const X beta = (X)(rr_new / rr);
     for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads) {
         const X pi = p[i];
         x[i] += alpha * pi;
@@ -165,10 +220,14 @@ __global__ __launch_bounds__(kVecThreads) void k_cg_xp(X* __restrict__ x, X* __r
     }
 }
 
-// K3 split in two around a reliable update: x += alpha p ...
+// K3 split in two around a reliable update: rr' as in K3; x += alpha p ...
 template <class X>
 __global__ __launch_bounds__(kVecThreads) void k_cg_x(X* __restrict__ x, const X* __restrict__ p, int64_t n,
-                                                      const CgScalars* __restrict__ sc, int par) {
+                                                      CgScalars* __restrict__ sc, int par,
+                                                      const double* __restrict__ part, unsigned m) {
+    __shared__ double s_part[kVecThreads / 64];
+    const double rr_new = sum_partials_all(part, m, s_part);
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->rr[par ^ 1] = rr_new;
     const X alpha = (X)(sc->rr[par] / sc->pq);
     for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads)
         x[i] += alpha * p[i];
@@ -300,8 +359,7 @@ int cg_run(const spc5_hip_mat& A, X* x, X* r, X* p, X* q, CgScalars* sc, double*
         if ((e = launch_spmv(A, p, q, t, ek1))) return e;
         hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kSumThreads), 0, t, part, A.plan.ntiles, &sc->pq);
         hipLaunchKernelGGL(k_cg_r<X>, dim3(g), dim3(kVecThreads), 0, t, r, q, n, sc, par, part);
-        hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kSumThreads), 0, t, part, (int64_t)g, &sc->rr[par ^ 1]);
-        hipLaunchKernelGGL(k_cg_xp<X>, dim3(g), dim3(kVecThreads), 0, t, x, p, r, n, sc, par);
+        hipLaunchKernelGGL(k_cg_xp<X>, dim3(g), dim3(kVecThreads), 0, t, x, p, r, n, sc, par, part, g);
         SPC5_HIP_TRY(hipGetLastError());
         return SPC5_HIP_OK;
     };
@@ -483,11 +541,10 @@ int mixed_cg_impl(const spc5_hip_mat& A64, const spc5_hip_mat& Alo, const double
         if ((e = spmv_lo(t))) return e;
         hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kSumThreads), 0, t, part, Alo.plan.ntiles, &sc->pq);
         hipLaunchKernelGGL(k_cg_r<XV>, dim3(g), dim3(kVecThreads), 0, t, r, q, n, sc, par, part);
-        hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kSumThreads), 0, t, part, (int64_t)g, &sc->rr[par ^ 1]);
         if (!last)
-            hipLaunchKernelGGL(k_cg_xp<XV>, dim3(g), dim3(kVecThreads), 0, t, d, p, r, n, sc, par);
+            hipLaunchKernelGGL(k_cg_xp<XV>, dim3(g), dim3(kVecThreads), 0, t, d, p, r, n, sc, par, part, g);
         else
-            hipLaunchKernelGGL(k_cg_x<XV>, dim3(g), dim3(kVecThreads), 0, t, d, p, n, sc, par);
+            hipLaunchKernelGGL(k_cg_x<XV>, dim3(g), dim3(kVecThreads), 0, t, d, p, n, sc, par, part, g);
         SPC5_HIP_TRY(hipGetLastError());
         return SPC5_HIP_OK;
     };
