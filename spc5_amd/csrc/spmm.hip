@@ -36,8 +36,33 @@ namespace {
 #endif
 using SpmmFloatAcc = std::conditional<SPC5_SPMM_FLOAT_ACC_DOUBLE != 0, double, float>::type;
 
-constexpr int kSpmmCap = 256;        // values expanded per batch
-constexpr int kSpmmAccBytes = 8192;  // LDS for the row x column sums of a tile
+// X rows requested together by one record group before the first use
+// waits: 16 with 64-column chunks (one group per wave, LDS-bound occupancy;
+// k = 64 on the 27-point stencil 5.17 -> 4.87 ms in double), 8 with narrower
+// chunks, where 16 measured slower (more registers, and groups with fewer
+// than 16 records per batch fall back to one request at a time).
+#ifndef SPC5_SPMM_U
+#define SPC5_SPMM_U 8
+#endif
+#ifndef SPC5_SPMM_U64
+#define SPC5_SPMM_U64 16
+#endif
+
+#ifndef SPC5_SPMM_CAP
+#define SPC5_SPMM_CAP 256
+#endif
+constexpr int kSpmmCap = SPC5_SPMM_CAP;  // values expanded per batch
+// a batch must hold the largest block (r = 8, VS = 16: 128 values)
+static_assert(kSpmmCap >= 128 && kSpmmCap % 64 == 0, "SPC5_SPMM_CAP: a multiple of 64, at least 128");
+// LDS for a tile's row x column sums; it sets the tile height (8 to 128
+// rows).  The loop waits on X rows, so occupancy matters most: on the
+// 27-point stencil 8192 bytes took 4.88 / 1.83 / 0.83 ms in double for
+// k = 64 / 16 / 4, 2048 bytes take 4.39 / 1.38 / 0.74 ms (1024 measured the
+// same as 2048, a 128-value batch cap slower in double).
+#ifndef SPC5_SPMM_ACC_BYTES
+#define SPC5_SPMM_ACC_BYTES 2048
+#endif
+constexpr int kSpmmAccBytes = SPC5_SPMM_ACC_BYTES;
 constexpr int kSpmmMaxKt = 64;       // columns per pass (one lane each)
 
 // One value of a batch: the value, where its X row is (a byte offset from
@@ -85,7 +110,8 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
     constexpr int K = kSpmmCap / 64;
     constexpr int NW = MaskWords<R * MB>::kWords;
     constexpr int G = 64 / KC;  // record groups per wave
-    constexpr int U = 8;        // records whose X rows are requested together
+    // records whose X rows are requested together
+    constexpr int U = KC == 64 ? SPC5_SPMM_U64 : SPC5_SPMM_U;
     using Rec = SpmmRec<ACC>;
     extern __shared__ __align__(16) unsigned char smem[];
     Rec* rec = reinterpret_cast<Rec*>(smem);                            // [kSpmmCap]
