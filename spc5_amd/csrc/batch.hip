@@ -8,7 +8,9 @@
 // same tile routine as the single-matrix kernel, with the matrix's own r
 // (chosen per matrix, e.g. by the autotuner) selected at run time.  When
 // every matrix has the same r, a kernel holding only that r's body runs
-// instead (the four bodies together need far more registers than one).
+// instead (the four bodies together need far more registers than one); with
+// r = 2 or 4 and row sorting switched on for every matrix, that kernel uses
+// the row-sorted tile (as the single-matrix product does).
 // The launch is captured once into a hipGraph; every product replays it.
 #include "spc5_mat.hpp"
 #include "spmv_tile.hpp"
@@ -39,6 +41,7 @@ struct spc5_hip_batch {
     int64_t ntiles = 0;
     int waves = 0;
     int only_r = 0;         // the r shared by every matrix, 0 if they differ
+    int sorted = 0;         // only_r is 2 or 4 and every matrix has row sorting on
     DeviceBuffer entries;   // BatchEntry[nmats]
     DeviceBuffer tile_mat;  // int32[ntiles]: matrix of each tile
     DeviceBuffer tile_loc;  // int32[ntiles]: tile index inside its matrix
@@ -68,9 +71,44 @@ union BatchSmem {
 
 // Waves per SIMD of the batch kernel: with every r body (ONLY_R = 0) the
 // most that compile without spills; with one r, the tile kernel's rule.
-constexpr int batch_waves_per_eu(int only_r, bool wide_x, int value_bytes) {
+constexpr int batch_waves_per_eu(int only_r, bool wide_x, int value_bytes, bool sorted = false) {
     // the persistent loop's own state costs one wave (two with 8-byte values)
-    return only_r == 0 ? 5 : tile_waves_per_eu(only_r, false, wide_x) - (value_bytes == 8 ? 2 : 1);
+    return only_r == 0 ? 5 : tile_waves_per_eu(only_r, sorted, wide_x) - (value_bytes == 8 ? 2 : 1);
+}
+
+// The batch loop over tiles: tile t of the list runs body(matrix entry,
+// tile bounds) on its wave.
+template <class Body>
+__device__ __forceinline__ void batch_tiles(const BatchEntry* __restrict__ mats, const int* __restrict__ tile_mat,
+                                            const int* __restrict__ tile_loc, int64_t ntiles, Body&& body) {
+    const int w = threadIdx.x >> 6;
+    const int64_t stride = (int64_t)gridDim.x * kWavesPerGroup;
+    // The tile index is the same on every lane (it comes from the wave index,
+    // which the compiler cannot tell is uniform): read it as a scalar, so the
+    // matrix descriptor is loaded into scalar registers.
+    const int64_t t0 = (int64_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWavesPerGroup + w));
+    for (int64_t t = t0; t < ntiles; t += stride) {
+        const BatchEntry& m = mats[tile_mat[t]];
+        const int lt = tile_loc[t];
+        body(m, m.tile_br[lt], m.tile_br[lt + 1], m.tile_val[lt], m.tile_val[lt + 1]);
+        wave_sync();
+    }
+}
+
+// Every matrix has r = R and row sorting on.
+template <class V, class X, class ACC, int VS, int MB, int R>
+__global__ __launch_bounds__(kWavesPerGroup * 64, batch_waves_per_eu(R, sizeof(X) > sizeof(ACC), sizeof(V), true)) void
+k_spmv_batch_sorted(const BatchEntry* __restrict__ mats, const int* __restrict__ tile_mat,
+                    const int* __restrict__ tile_loc, int64_t ntiles) {
+    static_assert(R == 2 || R == 4, "row-sorted tiles exist for r = 2 and 4");
+    __shared__ TileSmem<ACC, R, expand_cap<V>(), true> sm[kWavesPerGroup];
+    const int w = threadIdx.x >> 6;
+    const Epilogue ep;
+    batch_tiles(mats, tile_mat, tile_loc, ntiles, [&](const BatchEntry& m, int br0, int br1, int v0, int v1) {
+        spmv_tile<V, X, ACC, R, VS, MB, true>(sm[w], m.bptr, m.hdr, reinterpret_cast<const V*>(m.val),
+                                              reinterpret_cast<const X*>(m.x), reinterpret_cast<X*>(m.y), br0, br1,
+                                              v0, v1, m.nrows, m.ncols, ep);
+    });
 }
 
 template <class V, class X, class ACC, int VS, int MB, int ONLY_R>
@@ -120,8 +158,21 @@ void launch_batch_r(const spc5_hip_batch& b, hipStream_t s) {
                        b.ntiles);
 }
 
+template <class V, class X, class ACC, int VS, int MB, int R>
+void launch_batch_sorted(const spc5_hip_batch& b, hipStream_t s) {
+    const unsigned groups = (unsigned)((b.waves + kWavesPerGroup - 1) / kWavesPerGroup);
+    hipLaunchKernelGGL((k_spmv_batch_sorted<V, X, ACC, VS, MB, R>), dim3(groups), dim3(kWavesPerGroup * 64), 0, s,
+                       b.entries.as<const BatchEntry>(), b.tile_mat.as<const int>(), b.tile_loc.as<const int>(),
+                       b.ntiles);
+}
+
 template <class V, class X, class ACC, int VS, int MB>
 void launch_batch(const spc5_hip_batch& b, hipStream_t s) {
+    if (b.sorted) {
+        if (b.only_r == 2) launch_batch_sorted<V, X, ACC, VS, MB, 2>(b, s);
+        else launch_batch_sorted<V, X, ACC, VS, MB, 4>(b, s);
+        return;
+    }
     switch (b.only_r) {
     case 1: launch_batch_r<V, X, ACC, VS, MB, 1>(b, s); break;
     case 2: launch_batch_r<V, X, ACC, VS, MB, 2>(b, s); break;
@@ -132,9 +183,10 @@ void launch_batch(const spc5_hip_batch& b, hipStream_t s) {
 }
 
 // Waves per SIMD the batch's kernel runs at (sizes the persistent grid).
-int batch_kernel_waves(int value_type, int only_r) {
+int batch_kernel_waves(int value_type, int only_r, bool sorted) {
     const bool wide_x = false;  // batches use the matrices' own vector type
-    return batch_waves_per_eu(only_r, wide_x, value_type == SPC5_HIP_F64 ? 8 : value_type == SPC5_HIP_F32 ? 4 : 2);
+    return batch_waves_per_eu(only_r, wide_x, value_type == SPC5_HIP_F64 ? 8 : value_type == SPC5_HIP_F32 ? 4 : 2,
+                              sorted);
 }
 
 int launch_batch_any(const spc5_hip_batch& b, hipStream_t s) {
@@ -194,9 +246,13 @@ int spc5_hip_batch_create(int nmats, const spc5_hip_mat_t* mats, const void* con
     SPC5_HIP_TRY(hipGetDevice(&dev));
     SPC5_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     b->only_r = mats[0]->r;
-    for (int i = 1; i < nmats; ++i)
+    bool all_sorted = true;
+    for (int i = 0; i < nmats; ++i) {
         if (mats[i]->r != b->only_r) b->only_r = 0;
-    int64_t wv = waves > 0 ? waves : (int64_t)ncu * 4 * batch_kernel_waves(b->value_type, b->only_r);
+        all_sorted = all_sorted && mats[i]->row_sort != 0;
+    }
+    b->sorted = (b->only_r == 2 || b->only_r == 4) && all_sorted ? 1 : 0;
+    int64_t wv = waves > 0 ? waves : (int64_t)ncu * 4 * batch_kernel_waves(b->value_type, b->only_r, b->sorted != 0);
     if (wv > b->ntiles) wv = std::max<int64_t>(b->ntiles, 1);
     b->waves = (int)wv;
     SPC5_HIP_TRY(b->entries.alloc(sizeof(BatchEntry) * (size_t)nmats));

@@ -61,6 +61,51 @@ def test_batch_uneven_matrices(dtype, tol, same_r):
     b.close()
 
 
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 1e-5)])
+@pytest.mark.parametrize("r", [2, 4])
+def test_batch_row_sorted(dtype, tol, r):
+    """Every matrix with the same r in {2, 4} and row sorting on: the batch
+    runs the row-sorted tile, so each y is bit-identical to the matrix's own
+    row-sorted product (same tiles, same routine) and matches the oracle."""
+    from spc5_amd.batch import Batch
+    from spc5_amd.device import DeviceMatrix
+    rng = np.random.default_rng(11)
+    mats, xs, ys, singles, refs = [], [], [], [], []
+    for i in range(30):
+        n = int(rng.integers(1, 1500))
+        nc = int(rng.integers(1, 1200))
+        if i % 3 == 0:
+            rp, col, v = random_csr(n, nc, 0.03, seed=100 + i, dtype=dtype)
+        else:
+            rp, col, v = clustered_csr(n, nc, seed=100 + i, dtype=dtype, empty_frac=0.1)
+        m = DeviceMatrix.from_csr(dev(rp), dev(col), dev(v), r, ncols=nc)
+        m.row_sort = True
+        x = rng.uniform(-1, 1, nc).astype(dtype)
+        y0 = rng.uniform(-1, 1, n).astype(dtype)
+        ys1 = dev(y0)
+        m.spmv(dev(x), ys1)
+        singles.append(ys1)
+        mats.append(m)
+        xs.append(dev(x))
+        ys.append(dev(y0))
+        vs = O.VS_OF[np.dtype(dtype)]
+        bptr, hdr, vout, nb = O.convert(r, rp, col, v)
+        xp = np.zeros(nc + vs, dtype)
+        xp[:nc] = x
+        yr = np.zeros(n + 8, dtype)
+        if nb:
+            O.beta_spmv(r, n, bptr, hdr, vout, xp, yr)
+        refs.append((rp, v, x, y0, yr[:n].astype(np.float64)))
+    b = Batch(mats, xs, ys)
+    b.spmv(graph=True)
+    torch.cuda.synchronize()
+    for (rp, v, x, y0, yr), y, y1 in zip(refs, ys, singles):
+        got = y.cpu().numpy()
+        assert np.array_equal(got, y1.cpu().numpy())
+        assert normwise_error(got, y0.astype(np.float64) + yr, rp, v, x, y0) <= tol
+    b.close()
+
+
 @pytest.mark.parametrize("n,bw,fill,seed", [(1000, 4, 0.3, 7), (5003, 256, 0.3, 8), (777, 31, 0.9, 9)])
 def test_banded_generator_gpu_equals_host(n, bw, fill, seed):
     """The device generator of the uneven batch is byte-identical to the
