@@ -129,6 +129,44 @@ def test_cg_double(r):
     assert np.linalg.norm(xh - xs) / np.linalg.norm(xs) <= 1e-7
 
 
+def test_cg_many_tiles_matches_recurrence():
+    """A 5.29M-row Laplacian has more tiles (one p.q partial each) than one
+    pass of the partial-sum kernel covers (32,768): three CG iterations
+    equal the textbook recurrence run in torch on the same (separately
+    verified) product, to summation-order rounding."""
+    from spc5_amd import synth
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import cg
+    rp, col, val = synth.laplacian2d(2300)
+    n = len(rp) - 1
+    A = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val), 1)
+    assert n // 128 > 32768
+    b = dev(synth.uniform(n, 21))
+    x, st = cg(A, b, rtol=1e-14, maxit=3)
+    assert st["iterations"] == 3
+
+    def mul(v):
+        out = torch.zeros_like(v)
+        A.spmv(v, out)
+        return out
+
+    xr = torch.zeros_like(b)
+    r = b.clone()
+    p = r.clone()
+    rr = torch.dot(r, r)
+    for _ in range(3):
+        q = mul(p)
+        alpha = rr / torch.dot(p, q)
+        xr += alpha * p
+        r -= alpha * q
+        rr_new = torch.dot(r, r)
+        p = r + (rr_new / rr) * p
+        rr = rr_new
+    torch.cuda.synchronize()
+    assert float(torch.linalg.vector_norm(x - xr) / torch.linalg.vector_norm(xr)) <= 1e-12
+    assert st["relres"] == pytest.approx(float(torch.sqrt(rr) / torch.linalg.vector_norm(b)), rel=1e-9)
+
+
 @pytest.mark.parametrize("kind", ["f32", "bf16"])
 @pytest.mark.parametrize("r", (2, 4))
 def test_ir_cg_mixed(kind, r):
