@@ -9,9 +9,11 @@
 // value becomes a 16-byte LDS record (value, column, tile-local row).  The
 // wave then splits into 64/KC groups of KC lanes (one lane per column of
 // the chunk); each group walks a contiguous run of the batch's records,
-// requests the X rows of 8 records at a time (one contiguous kt-wide row
-// segment per record) and keeps the running row sum in a register, adding
-// it to the tile's LDS row x column sums only when the row changes.  With
+// requests the X rows of U records at a time (one contiguous kt-wide row
+// segment per record; with 64-column chunks the whole wave is one group and
+// the records are broadcast to scalar registers) and keeps the running row
+// sum in a register, adding it to the tile's LDS row x column sums only when
+// the row changes.  With
 // kt = k the matrix is fetched once for all k columns; smaller kt re-reads
 // the matrix k/kt times with a smaller X working set (the autotuner picks
 // kt).
@@ -51,9 +53,21 @@ using SpmmFloatAcc = std::conditional<SPC5_SPMM_FLOAT_ACC_DOUBLE != 0, double, f
 #ifndef SPC5_SPMM_CAP
 #define SPC5_SPMM_CAP 256
 #endif
-constexpr int kSpmmCap = SPC5_SPMM_CAP;  // values expanded per batch
+#ifndef SPC5_SPMM_CAP64_NARROW
+#define SPC5_SPMM_CAP64_NARROW 128
+#endif
+// Values expanded per batch.  64-column chunks of 4- or 2-byte X use the
+// smaller cap: their records are scalars, LDS then limits occupancy, and
+// k = 64 float on the 27-point stencil went 2.99 -> 2.59 ms (double and
+// narrower chunks measured slower with it).
+template <int KC, class X>
+constexpr int spmm_cap() {
+    return KC == 64 && sizeof(X) < 8 ? SPC5_SPMM_CAP64_NARROW : SPC5_SPMM_CAP;
+}
 // a batch must hold the largest block (r = 8, VS = 16: 128 values)
-static_assert(kSpmmCap >= 128 && kSpmmCap % 64 == 0, "SPC5_SPMM_CAP: a multiple of 64, at least 128");
+static_assert(SPC5_SPMM_CAP >= 128 && SPC5_SPMM_CAP % 64 == 0, "SPC5_SPMM_CAP: a multiple of 64, at least 128");
+static_assert(SPC5_SPMM_CAP64_NARROW >= 128 && SPC5_SPMM_CAP64_NARROW % 64 == 0,
+              "SPC5_SPMM_CAP64_NARROW: a multiple of 64, at least 128");
 // LDS for a tile's row x column sums; it sets the tile height (8 to 128
 // rows).  The loop waits on X rows, so occupancy matters most: on the
 // 27-point stencil 8192 bytes took 4.88 / 1.83 / 0.83 ms in double for
@@ -83,9 +97,10 @@ int spmm_max_rows(int kt) {
     return rows;
 }
 
-template <class ACC>
+template <class ACC, int KC, class X>
 size_t spmm_smem_bytes(int maxrows, int kt, int r) {
-    return sizeof(SpmmRec<ACC>) * kSpmmCap + sizeof(ACC) * (size_t)maxrows * kt + sizeof(int) * (maxrows / r + 1);
+    return sizeof(SpmmRec<ACC>) * spmm_cap<KC, X>() + sizeof(ACC) * (size_t)maxrows * kt +
+           sizeof(int) * (maxrows / r + 1);
 }
 
 template <bool WIDE, class X>
@@ -98,6 +113,17 @@ __device__ __forceinline__ X spmm_x(const X* __restrict__ xcol, const X* __restr
     }
 }
 
+// The value of lane l (a compile-time or uniform lane index) on every lane.
+__device__ __forceinline__ float read_lane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double read_lane(double v, int l) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // WIDE: X spans 4 GiB or more (or rows of 16 MiB or more), so records hold
 // columns and X is addressed with 64-bit arithmetic; otherwise they hold
 // 32-bit byte offsets of the X rows, computed with 24-bit multiplies.
@@ -107,15 +133,16 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
                                                     X* __restrict__ y, int64_t ldy, int k, int kt,
                                                     const int* __restrict__ tile_br, const int* __restrict__ tile_val,
                                                     int64_t ntiles, int64_t nrows, int maxrows, ACC alpha, ACC beta) {
-    constexpr int K = kSpmmCap / 64;
+    constexpr int CAP = spmm_cap<KC, X>();
+    constexpr int K = CAP / 64;
     constexpr int NW = MaskWords<R * MB>::kWords;
     constexpr int G = 64 / KC;  // record groups per wave
     // records whose X rows are requested together
     constexpr int U = KC == 64 ? SPC5_SPMM_U64 : SPC5_SPMM_U;
     using Rec = SpmmRec<ACC>;
     extern __shared__ __align__(16) unsigned char smem[];
-    Rec* rec = reinterpret_cast<Rec*>(smem);                            // [kSpmmCap]
-    ACC* acc = reinterpret_cast<ACC*>(rec + kSpmmCap);                  // [maxrows * kt]
+    Rec* rec = reinterpret_cast<Rec*>(smem);                            // [CAP]
+    ACC* acc = reinterpret_cast<ACC*>(rec + CAP);                       // [maxrows * kt]
     int* rs = reinterpret_cast<int*>(acc + (size_t)maxrows * kt);       // [maxrows / R + 1]
 
     const int64_t t = blockIdx.x;
@@ -160,7 +187,7 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
 #pragma unroll
         for (int w = 0; w < NW; ++w) cnt += __popcll(mw_cur[w]);
         const int incl = wave_incl_scan(cnt);
-        const int ntake = __popcll(ballot64(valid && incl <= kSpmmCap));
+        const int ntake = __popcll(ballot64(valid && incl <= CAP));
         const int total = __builtin_amdgcn_readlane(incl, ntake - 1);
         ACC a[K];
         load_values<V, ACC, K>(val, v, total, lane, a);
@@ -197,23 +224,56 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
         const int qb = q + per < total ? q + per : total;
         ACC run = (ACC)0;
         int prow = -1;
-        for (; q + U <= qb; q += U) {
-            Rec rr[U];
+        if constexpr (KC == 64 && !WIDE) {
+            // One group: every record is the same on all lanes.  Lane u reads
+            // record q + u (one LDS read for U records) and the fields are
+            // broadcast to scalars, so the records take no vector registers
+            // and each X row request is a scalar base plus the lane's offset.
+            static_assert(U <= 64 && (U & (U - 1)) == 0, "U: a power of two up to 64");
+            for (; q + U <= qb; q += U) {
+                const Rec mine = rec[q + (lane & (U - 1))];
+                unsigned cu[U];
+                int ru[U];
+                ACC vu[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) rr[u] = rec[q + u];
-            // All U record reads are issued before the first use waits.
-            __builtin_amdgcn_sched_barrier(0);
-            ACC xv[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) xv[u] = (ACC)spmm_x<WIDE>(xcol, x, rr[u].col, ldx, lane_off);
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (rr[u].row != prow) {
-                    if (prow >= 0 && colok) atomicAdd(&acc[prow * kt + jl], run);
-                    run = (ACC)0;
-                    prow = rr[u].row;
+                for (int u = 0; u < U; ++u) {
+                    cu[u] = (unsigned)__builtin_amdgcn_readlane(mine.col, u);
+                    ru[u] = __builtin_amdgcn_readlane(mine.row, u);
+                    vu[u] = read_lane(mine.v, u);
                 }
-                run += rr[u].v * xv[u];
+                ACC xv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    xv[u] = (ACC) * reinterpret_cast<const X*>(reinterpret_cast<const char*>(x) + cu[u] + lane_off);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (ru[u] != prow) {
+                        if (prow >= 0 && colok) atomicAdd(&acc[prow * kt + jl], run);
+                        run = (ACC)0;
+                        prow = ru[u];
+                    }
+                    run += vu[u] * xv[u];
+                }
+            }
+        } else {
+            for (; q + U <= qb; q += U) {
+                Rec rr[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) rr[u] = rec[q + u];
+                // All U record reads are issued before the first use waits.
+                __builtin_amdgcn_sched_barrier(0);
+                ACC xv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) xv[u] = (ACC)spmm_x<WIDE>(xcol, x, rr[u].col, ldx, lane_off);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (rr[u].row != prow) {
+                        if (prow >= 0 && colok) atomicAdd(&acc[prow * kt + jl], run);
+                        run = (ACC)0;
+                        prow = rr[u].row;
+                    }
+                    run += rr[u].v * xv[u];
+                }
             }
         }
         for (; q < qb; ++q) {
@@ -253,7 +313,7 @@ template <class V, class X, class ACC, int VS, int MB, int KC, bool WIDE>
 int launch_spmm_kc(const spc5_hip_mat& m, const Plan& p, int maxrows, const void* x, int64_t ldx, void* y,
                    int64_t ldy, int k, int kt, double alpha, double beta, hipStream_t s) {
     const dim3 grid((unsigned)p.ntiles, (unsigned)((k + kt - 1) / kt));
-    const size_t shm = spmm_smem_bytes<ACC>(maxrows, kt, m.r);
+    const size_t shm = spmm_smem_bytes<ACC, KC, X>(maxrows, kt, m.r);
     const int* tb = p.tile_br->as<int>();
     const int* tv = p.tile_val->as<int>();
     const V* val = reinterpret_cast<const V*>(m.values);
