@@ -10,8 +10,9 @@
 // wave then splits into 64/KC groups of KC lanes (one lane per column of
 // the chunk); each group walks a contiguous run of the batch's records,
 // requests the X rows of U records at a time (one contiguous kt-wide row
-// segment per record; with 64-column chunks the whole wave is one group and
-// the records are broadcast to scalar registers) and keeps the running row
+// segment per record; with 64-column chunks of float or bfloat16 X the
+// whole wave is one group and the records are broadcast to scalar
+// registers) and keeps the running row
 // sum in a register, adding it to the tile's LDS row x column sums only when
 // the row changes.  With
 // kt = k the matrix is fetched once for all k columns; smaller kt re-reads
@@ -58,8 +59,10 @@ using SpmmFloatAcc = std::conditional<SPC5_SPMM_FLOAT_ACC_DOUBLE != 0, double, f
 #endif
 // Values expanded per batch.  64-column chunks of 4- or 2-byte X use the
 // smaller cap: their records are scalars, LDS then limits occupancy, and
-// k = 64 float on the 27-point stencil went 2.99 -> 2.59 ms (double and
-// narrower chunks measured slower with it).
+// k = 64 float on the 27-point stencil went 2.99 -> 2.59 ms (narrower
+// chunks measured slower with it).  Double X keeps vector records: the
+// scalar form measured 2 % slower there (its 512-byte X rows, not
+// occupancy, set the pace).
 template <int KC, class X>
 constexpr int spmm_cap() {
     return KC == 64 && sizeof(X) < 8 ? SPC5_SPMM_CAP64_NARROW : SPC5_SPMM_CAP;
@@ -224,7 +227,7 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
         const int qb = q + per < total ? q + per : total;
         ACC run = (ACC)0;
         int prow = -1;
-        if constexpr (KC == 64 && !WIDE) {
+        if constexpr (KC == 64 && !WIDE && sizeof(X) < 8) {
             // One group: every record is the same on all lanes.  Lane u reads
             // record q + u (one LDS read for U records) and the fields are
             // broadcast to scalars, so the records take no vector registers
