@@ -49,7 +49,10 @@ struct CgScalars {
 };
 
 constexpr int kVecThreads = 256;
-constexpr int kMaxVecGroups = 2048;
+#ifndef SPC5_VEC_GROUPS
+#define SPC5_VEC_GROUPS 2048
+#endif
+constexpr int kMaxVecGroups = SPC5_VEC_GROUPS;  // most workgroups of a vector kernel (and partials)
 
 __device__ __forceinline__ double block_sum(double v, double* s_part) {
     v = wave_sum_f64(v);
