@@ -297,6 +297,15 @@ int spc5_hip_synth_banded_count(int64_t n, int64_t bw, uint64_t thr, uint64_t se
 int spc5_hip_synth_banded_fill(int64_t n, int64_t bw, uint64_t thr, uint64_t seed, const int* d_rowptr, int* d_col,
                                double* d_val, void* stream);
 
+/* Read-dominated streaming probe (no counterpart in the reference; a
+ * measurement aid): reads read_bytes of d_src with non-temporal 16-byte
+ * loads over `grid` workgroups of 256 threads and writes write_bytes of
+ * d_dst (which must hold at least 16 bytes even when write_bytes is 0).
+ * Sizes are multiples of 16.  With a write share of about 3 % it has the
+ * HBM mix of one SpMV, so its rate is the ceiling such a kernel can reach. */
+int spc5_hip_stream_probe(const void* d_src, int64_t read_bytes, void* d_dst, int64_t write_bytes, int grid,
+                          void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* 3. Host-pointer helpers (used by include/spc5_hip.hpp)                 */
 /* ---------------------------------------------------------------------- */

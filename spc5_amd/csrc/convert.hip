@@ -62,13 +62,20 @@ struct Cursors {
     }
 };
 
+// Blocks per block-row.  Also checks the precondition of the format
+// (src/spc5.hpp:165-228 assumes it without a check): columns are
+// non-negative and strictly increasing inside every row.  A violation sets
+// bit 0 of *err; the fill pass is then never run.
 template <int R, int VS>
 __global__ void k_convert_count(int64_t nrows, int64_t nbr, const int* __restrict__ rowptr,
-                                const int* __restrict__ col, int* __restrict__ cnt_out) {
+                                const int* __restrict__ col, int* __restrict__ cnt_out, int* __restrict__ err) {
     const int64_t br = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (br >= nbr) return;
     Cursors<R> c;
     c.init(br * R, nrows, rowptr, col);
+    bool bad = false;
+#pragma unroll
+    for (int s = 0; s < R; ++s) bad |= c.head[s] < 0;
     int nb = 0;
     int start;
     while (c.next_start(start)) {
@@ -77,12 +84,15 @@ __global__ void k_convert_count(int64_t nrows, int64_t nbr, const int* __restric
 #pragma unroll
         for (int s = 0; s < R; ++s) {
             while (c.cur[s] < c.end[s] && c.head[s] < lim) {
+                const int prev = c.head[s];
                 ++c.cur[s];
                 c.head[s] = c.cur[s] < c.end[s] ? col[c.cur[s]] : INT_MAX;
+                bad |= c.cur[s] < c.end[s] && c.head[s] <= prev;
             }
         }
     }
     cnt_out[br] = nb;
+    if (bad) atomicOr(err, 1);
 }
 
 template <class W, int R, int VS, int MB>
@@ -110,7 +120,10 @@ __global__ void k_convert_fill(int64_t nrows, int64_t nbr, const int* __restrict
 #pragma unroll
         for (int s = 0; s < R; ++s) {
             while (c.cur[s] < c.end[s] && c.head[s] < lim) {
-                const int bit = s * VS + (c.head[s] - start);
+                // In range for sorted columns (checked by the count pass); the
+                // guard only keeps a direct call on unsorted input in bounds.
+                const unsigned off = (unsigned)(c.head[s] - start);
+                const int bit = s * VS + (int)(off < (unsigned)VS ? off : 0u);
                 mw[bit >> 6] |= 1ull << (bit & 63);
                 vout[vo++] = vin[c.cur[s]];
                 ++c.cur[s];
@@ -125,9 +138,10 @@ __global__ void k_convert_fill(int64_t nrows, int64_t nbr, const int* __restrict
 }
 
 template <int R, int VS>
-void launch_count(int64_t nrows, int64_t nbr, const int* rowptr, const int* col, int* out, hipStream_t s) {
+void launch_count(int64_t nrows, int64_t nbr, const int* rowptr, const int* col, int* out, int* err,
+                  hipStream_t s) {
     const unsigned g = (unsigned)((nbr + 255) / 256);
-    hipLaunchKernelGGL((k_convert_count<R, VS>), dim3(g), dim3(256), 0, s, nrows, nbr, rowptr, col, out);
+    hipLaunchKernelGGL((k_convert_count<R, VS>), dim3(g), dim3(256), 0, s, nrows, nbr, rowptr, col, out, err);
 }
 
 template <class W, int VS, int MB>
@@ -208,17 +222,21 @@ int spc5_hip_convert_count(int vs, int r, int64_t nrows, const int* d_rowptr, co
     int rc = check_rowptr(d_rowptr, nrows, s);
     if (rc) return rc;
     SPC5_HIP_TRY(hipMemsetAsync(d_rowsSize_out + nbr, 0, sizeof(int), s));
+    DeviceBuffer err;
+    SPC5_HIP_TRY(err.alloc(sizeof(int)));
+    SPC5_HIP_TRY(hipMemsetAsync(err.p, 0, sizeof(int), s));
     if (nbr > 0) {
+        SPC5_CHECK_ARG(d_colidx != nullptr || nrows == 0, "colidx is null");
         const int key = r * 100 + vs;
         switch (key) {
-        case 108: launch_count<1, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
-        case 208: launch_count<2, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
-        case 408: launch_count<4, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
-        case 808: launch_count<8, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
-        case 116: launch_count<1, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
-        case 216: launch_count<2, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
-        case 416: launch_count<4, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
-        case 816: launch_count<8, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, s); break;
+        case 108: launch_count<1, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
+        case 208: launch_count<2, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
+        case 408: launch_count<4, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
+        case 808: launch_count<8, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
+        case 116: launch_count<1, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
+        case 216: launch_count<2, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
+        case 416: launch_count<4, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
+        case 816: launch_count<8, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
         }
         SPC5_HIP_TRY(hipGetLastError());
     }
@@ -226,9 +244,14 @@ int spc5_hip_convert_count(int vs, int r, int64_t nrows, const int* d_rowptr, co
     SPC5_HIP_TRY(total.alloc(sizeof(int)));
     rc = scan_exclusive_i32(d_rowsSize_out, nbr + 1, total.as<int>(), s);
     if (rc) return rc;
-    int h = 0;
+    int h = 0, herr = 0;
     SPC5_HIP_TRY(hipMemcpyAsync(&h, total.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPC5_HIP_TRY(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
     SPC5_HIP_TRY(hipStreamSynchronize(s));
+    if (herr) {
+        set_error("CSR columns must be non-negative and strictly increasing inside each row");
+        return SPC5_HIP_ERR_INVALID;
+    }
     *nblocks = h;
     return SPC5_HIP_OK;
 }

@@ -185,6 +185,8 @@ __global__ __launch_bounds__(kVecThreads) void k_cg_r(X* __restrict__ r, const X
 __device__ __forceinline__ double sum_partials_all(const double* __restrict__ part, unsigned m,
                                                    double* __restrict__ s_part) {
     constexpr int kLoads = kMaxVecGroups / kVecThreads;
+    static_assert(kMaxVecGroups % kVecThreads == 0 && kMaxVecGroups >= kVecThreads,
+                  "SPC5_VEC_GROUPS must be a positive multiple of the vector workgroup size");
     double v[kLoads];
 #pragma unroll
     for (int k = 0; k < kLoads; ++k) {
@@ -474,7 +476,10 @@ int mixed_cg_impl(const spc5_hip_mat& A64, const spc5_hip_mat& Alo, const double
     constexpr bool kF64 = sizeof(XV) == 8;
     const int64_t n = A64.nrows;
     const int check = 8;
-    const int64_t npart = std::max<int64_t>(kMaxVecGroups, Alo.plan.ntiles);
+    // Partials: one per workgroup of a vector kernel, one per tile of the
+    // float product, and one per tile of the double product of the tail
+    // (cg_run on A64), whose tile list can be longer when A64 has another r.
+    const int64_t npart = std::max<int64_t>(kMaxVecGroups, std::max(Alo.plan.ntiles, A64.plan.ntiles));
     DeviceBuffer vr64, vr, vp, vq, vd, vsc, vtmp, vpart;
     if (!kF64) {
         SPC5_HIP_TRY(vr64.alloc(sizeof(double) * (size_t)n));

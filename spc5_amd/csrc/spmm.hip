@@ -400,6 +400,13 @@ int spc5_hip_spmm(spc5_hip_mat_t h, int k, int kt, double alpha, const void* d_x
     if (kt <= 0 || kt > k) kt = k;
     if (kt > kSpmmMaxKt) kt = kSpmmMaxKt;
     hipStream_t s = as_stream(stream);
+    if (k == 1 && ldx == 1 && ldy == 1) {
+        // One contiguous column: the single-vector product.
+        Epilogue ep;
+        ep.alpha = alpha;
+        ep.beta = beta;
+        return launch_spmv(*h, d_x, d_y, s, ep);
+    }
     if (h->value_type == SPC5_HIP_F64)
         return launch_spmm_t<double, double, double, 8, 1>(*h, d_x, ldx, d_y, ldy, k, kt, alpha, beta, s);
     if (h->value_type == SPC5_HIP_F32 && h->vs == 16)

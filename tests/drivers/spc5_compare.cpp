@@ -5,7 +5,7 @@
 // prepared split/Spmv_omp pair), with the accuracy printed in the
 // reference's words plus a normwise error used by tests/test_gpu_cpp_dropin.py.
 //
-// usage: spc5_compare <dense N | lap N | mm FILE> <double|float>
+// usage: spc5_compare <dense N | lap N | mm FILE | api FILE> <double|float>
 #include "spc5_hip.hpp"
 
 #include <chrono>
@@ -135,13 +135,68 @@ static int run(std::vector<Ijv<T>> values, int nrows, int ncols) {
     return fails ? 1 : 0;
 }
 
+// API check on a CSR matrix stored raw (int64 nrows, ncols, nnz; int32
+// rowptr[nrows+1]; int32 col[nnz]; double values[nnz]): prints the choices
+// of SPC5_find_best (and SPC5_find_best_omp<double> for 8 and 36 threads),
+// and writes, for every format, the (row, column, value) triplets that
+// SPC5_iterate visits to FILE.iter_r<r>.bin (int32 row, int32 column,
+// double value each) for tests/test_gpu_cpp_dropin.py to compare.
+template <class T>
+static int api_check(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot open " + path);
+    int64_t hdr[3];
+    f.read(reinterpret_cast<char*>(hdr), sizeof(hdr));
+    const int nrows = (int)hdr[0], ncols = (int)hdr[1], nnz = (int)hdr[2];
+    std::vector<int> rp(nrows + 1), col(nnz);
+    std::vector<double> v(nnz);
+    f.read(reinterpret_cast<char*>(rp.data()), sizeof(int) * rp.size());
+    f.read(reinterpret_cast<char*>(col.data()), sizeof(int) * col.size());
+    f.read(reinterpret_cast<char*>(v.data()), sizeof(double) * v.size());
+    if (!f) throw std::runtime_error("short file " + path);
+    std::vector<Ijv<T>> ijv;
+    for (int i = 0; i < nrows; ++i)
+        for (int p = rp[i]; p < rp[i + 1]; ++p) ijv.push_back({i, col[p], T(v[p])});
+    std::cout.precision(17);
+    {
+        SPC5Mat<T> csr = COO_sorted_to_CSR<T>(nrows, ncols, ijv.data(), nnz);
+        auto best = SPC5_find_best<T>(csr);
+        std::cout << "FINDBEST " << spc5_hip_detail::r_of(best.first) << " " << best.second << "\n";
+        if constexpr (sizeof(T) == sizeof(double)) {
+            for (int nth : {8, 36}) {
+                auto bo = SPC5_find_best_omp<double>(csr, nth);
+                std::cout << "FINDBEST_OMP " << nth << " " << spc5_hip_detail::r_of(bo.first) << " " << bo.second
+                          << "\n";
+            }
+        }
+    }
+    const SPC5_MATRIX_TYPE fmts[4] = {FORMAT_1rVc_WT, FORMAT_2rVc, FORMAT_4rVc, FORMAT_8rVc};
+    for (SPC5_MATRIX_TYPE fmt : fmts) {
+        SPC5Mat<T> mat = COO_sorted_to_CSR<T>(nrows, ncols, ijv.data(), nnz);
+        CSR_to_SPC5<T>(&mat, fmt);
+        std::ofstream o(path + ".iter_r" + std::to_string(spc5_hip_detail::r_of(fmt)) + ".bin", std::ios::binary);
+        long long count = 0;
+        SPC5_iterate<T>(mat, [&](int row, int c, T val) {
+            const double d = (double)val;
+            o.write(reinterpret_cast<const char*>(&row), sizeof(int));
+            o.write(reinterpret_cast<const char*>(&c), sizeof(int));
+            o.write(reinterpret_cast<const char*>(&d), sizeof(double));
+            ++count;
+        });
+        std::cout << "ITERATE " << spc5_hip_detail::r_of(fmt) << " " << count << "\n";
+    }
+    std::cout << "RESULT DONE" << std::endl;
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc < 4) {
-        std::cerr << "usage: " << argv[0] << " <dense N | lap N | mm FILE> <double|float>\n";
+        std::cerr << "usage: " << argv[0] << " <dense N | lap N | mm FILE | api FILE> <double|float>\n";
         return 2;
     }
     const std::string kind = argv[1], arg = argv[2], real = argv[3];
     try {
+        if (kind == "api") return real == "double" ? api_check<double>(arg) : api_check<float>(arg);
         int nrows = 0, ncols = 0;
         if (real == "double") {
             std::vector<Ijv<double>> v;

@@ -17,6 +17,13 @@ Outputs (data only: inputs and the reference's outputs):
   and SHA-256 digests of the reference converter's rowsSize / headers /
   values for beta(1,8) and beta(4,8) double, and of the reference kernel's y
   for x = U[-1,1] (splitmix64 seed 0) and y0 = 0.
+* ``big_digests.json`` -- block counts and SHA-256 digests of the reference
+  converter's output at full size: the clustered-runs matrix (4,194,304
+  rows, double beta(r,8) and float beta(r,16), every r) and the 27-point
+  stencil 160^3 (double beta(2,8), beta(4,8)).
+
+``python tests/golden/make_golden.py [small] [lap512] [big]`` writes a
+subset.
 """
 from __future__ import annotations
 
@@ -36,6 +43,10 @@ import oracle as O  # noqa: E402
 from matgen import clustered_csr, dense_csr, laplacian2d, random_csr  # noqa: E402
 
 RS = (1, 2, 4, 8)
+# Seed of each small case's x and y0 (fixed, so that rerunning this script
+# rewrites the same vectors).
+SEEDS = {"lap8": 1001, "dense16": 1002, "rand37x70": 1003, "clustered101x300": 1004, "rows13": 1005,
+         "onerow200": 1006, "empty5": 1007, "dense_runs64": 1008}
 
 
 def cases():
@@ -59,7 +70,7 @@ def make_small():
     for name, (rp, col, val64), ncols in cases():
         out = {"rowptr": rp, "col": col, "val64": val64.astype(np.float64), "ncols": np.int64(ncols)}
         n = len(rp) - 1
-        rng = np.random.default_rng(abs(hash(name)) % (2 ** 31))
+        rng = np.random.default_rng(SEEDS[name])
         x64 = np.zeros(ncols + 15)
         x64[:ncols] = rng.uniform(-1, 1, ncols)
         y064 = np.zeros(n + 7)
@@ -122,8 +133,53 @@ def make_lap512():
     print("wrote lap512", flush=True)
 
 
+def make_big():
+    """Full-size digests of the reference converter on the two large
+    workloads: the clustered-runs matrix (N = 4,194,304, double beta(r,8)
+    and float beta(r,16), every r) and the 27-point stencil 160^3 (double
+    beta(2,8) and beta(4,8)).  rowsSize is hashed over its first
+    ceil(N/r)+1 entries (the block-row pointers; the reference leaves the
+    rest of the N+1 array as it was)."""
+    from spc5_amd import synth
+    res = {}
+    rp, col, val = synth.clustered_runs(4194304, seed=1, value_seed=2)
+    n = len(rp) - 1
+    cl = {"n": n, "nnz": int(rp[-1]), "generator": "clustered_runs(4194304, seed=1, value_seed=2)"}
+    for tag, dt in (("f64", np.float64), ("f32", np.float32)):
+        v = val.astype(dt)
+        for r in RS:
+            rs, hdr, vout, nb = O.ref_convert(r, rp, col, v)
+            nbr = (n + r - 1) // r
+            cl[f"{tag}_r{r}"] = {"nblocks": nb, "rowsSize_sha256": sha(rs[: nbr + 1]), "headers_sha256": sha(hdr),
+                                 "values_sha256": sha(vout)}
+            print("clustered", tag, r, nb, flush=True)
+            del rs, hdr, vout
+        del v
+    res["clustered"] = cl
+    del rp, col, val
+    rp, col, val = synth.stencil27(160)
+    n = len(rp) - 1
+    st = {"n": n, "nnz": int(rp[-1]), "generator": "stencil27(160)"}
+    for r in (2, 4):
+        rs, hdr, vout, nb = O.ref_convert(r, rp, col, val)
+        nbr = (n + r - 1) // r
+        st[f"f64_r{r}"] = {"nblocks": nb, "rowsSize_sha256": sha(rs[: nbr + 1]), "headers_sha256": sha(hdr),
+                           "values_sha256": sha(vout)}
+        print("stencil27", r, nb, flush=True)
+        del rs, hdr, vout
+    res["stencil27"] = st
+    with open(os.path.join(HERE, "big_digests.json"), "w") as f:
+        json.dump(res, f, indent=1)
+    print("wrote big_digests", flush=True)
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("oracle/_ref/libspc5_ref.so is missing: run `make -C oracle` where the reference tree exists")
-    make_small()
-    make_lap512()
+    which = sys.argv[1:] or ["small", "lap512", "big"]
+    if "small" in which:
+        make_small()
+    if "lap512" in which:
+        make_lap512()
+    if "big" in which:
+        make_big()

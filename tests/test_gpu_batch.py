@@ -116,3 +116,69 @@ def test_banded_generator_gpu_equals_host(n, bw, fill, seed):
     assert np.array_equal(drp.cpu().numpy(), rp)
     assert np.array_equal(dcol.cpu().numpy(), col)
     assert np.array_equal(dval.cpu().numpy().view(np.uint64), val.view(np.uint64))
+
+
+def _oracle_y(r, rp, col, v, x, y0, vs, msz):
+    """y0 + A x by the oracle's beta(r,VS) product (reference arithmetic)."""
+    bptr, hdr, vout, nb = O.convert(r, rp, col, v, vs=vs, msz=msz)
+    n, nc = len(rp) - 1, len(x)
+    xp = np.zeros(nc + vs, v.dtype)
+    xp[:nc] = x
+    yr = np.zeros(n + 8, v.dtype)
+    if nb:
+        O.beta_spmv(r, n, bptr, hdr, vout, xp, yr, vs=vs, msz=msz)
+    return y0.astype(np.float64) + yr[:n].astype(np.float64)
+
+
+@pytest.mark.parametrize("kind", ("f64", "lowp_f32"))
+@pytest.mark.parametrize("r_rule", ("autotuned", "mixed"))
+def test_batch_banded_recipe(kind, r_rule):
+    """The uneven batch's own recipe at reduced size: 60 banded matrices
+    with the batch's (rows, bandwidth, seed) table, rows scaled down 40x
+    (25 .. 5,000), bandwidth capped below the row count, 30 % fill, each with
+    its own x.  r per matrix from the autotuner's model ("autotuned") or
+    forced to cycle 1, 2, 4, 8 ("mixed": the kernel with every r's body);
+    double values, or float copies of the double matrices (beta(r,8),
+    float sums) as the benchmark times them.  One hipGraph launch, replayed
+    three times with eager work between, every y against the oracle
+    (normwise 1e-12 double, 1e-5 float)."""
+    from spc5_amd import autotune, synth
+    from spc5_amd.batch import Batch
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import lowp
+    rng = np.random.default_rng(21)
+    mats, xs, ys, cases = [], [], [], []
+    rs_used = set()
+    for i, (n, bw, seed) in enumerate(synth.uneven_batch_params(60)):
+        n = max(25, n // 40)
+        bw = min(bw, n - 1)
+        drp, dcol, dval = synth.banded_gpu(n, bw, 0.3, seed)
+        if r_rule == "autotuned":
+            r, _ = autotune.choose_r(drp, dcol, dval, mode="model")
+        else:
+            r = (1, 2, 4, 8)[i % 4]
+        rs_used.add(r)
+        m = DeviceMatrix.from_csr(drp, dcol, dval, r)
+        rp, col, v = drp.cpu().numpy(), dcol.cpu().numpy(), dval.cpu().numpy()
+        if kind == "lowp_f32":
+            m = lowp(m, "f32")
+            v = v.astype(np.float32)
+        x = rng.uniform(-1, 1, n).astype(v.dtype)
+        y0 = rng.uniform(-1, 1, n).astype(v.dtype)
+        mats.append(m)
+        xs.append(dev(x))
+        ys.append(dev(y0))
+        cases.append((r, rp, col, v, x, y0))
+    if r_rule == "mixed":
+        assert rs_used == {1, 2, 4, 8}
+    b = Batch(mats, xs, ys)
+    tol = 1e-12 if kind == "f64" else 1e-5
+    for rep in range(3):
+        b.spmv(graph=True)
+        torch.cuda.synchronize()
+        for (r, rp, col, v, x, y0), y in zip(cases, ys):
+            # y0 + (rep+1) A x: the product of the matrix scaled by rep+1
+            want = _oracle_y(r, rp, col, v * (rep + 1), x, y0, 8, 1)
+            assert normwise_error(y.cpu().numpy(), want, rp, v * (rep + 1), x, y0) <= tol, (rep, r, len(rp) - 1)
+        torch.ones(1000, device="cuda").sum().item()
+    b.close()

@@ -43,6 +43,106 @@ def test_spmm(dtype, tol, r, k, kt):
     assert err <= tol, err
 
 
+def _rounded(val, kind):
+    """Values as the reduced-precision copy stores them (round to nearest
+    even; bfloat16 through float), widened back to double."""
+    t = torch.from_numpy(np.asarray(val, np.float64))
+    if kind == "lowp_f32":
+        t = t.to(torch.float32)
+    elif kind == "lowp_bf16":
+        t = t.to(torch.float32).to(torch.bfloat16)
+    return t.to(torch.float64).numpy()
+
+
+def _matrix_of_kind(kind, rp, col, val64, r, ncols):
+    """f64: double beta(r,8); f32: float beta(r,16) (the reference float
+    format); lowp_f32 / lowp_bf16: float / bfloat16 copies of the double
+    beta(r,8) matrix (the precision form's value views).  Returns the
+    matrix, the vector dtype and the values it holds (as double)."""
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import lowp
+    if kind == "f64":
+        return DeviceMatrix.from_csr(dev(rp), dev(col), dev(val64), r, ncols=ncols), np.float64, val64
+    if kind == "f32":
+        v32 = val64.astype(np.float32)
+        return DeviceMatrix.from_csr(dev(rp), dev(col), dev(v32), r, ncols=ncols), np.float32, v32.astype(np.float64)
+    A64 = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val64), r, ncols=ncols)
+    A = lowp(A64, "f32" if kind == "lowp_f32" else "bf16")  # the view keeps A64 alive
+    return A, np.float32, _rounded(val64, kind)
+
+
+@pytest.mark.parametrize("kind", ("f64", "f32", "lowp_f32", "lowp_bf16"))
+@pytest.mark.parametrize("r", (1, 2, 4, 8))
+@pytest.mark.parametrize("k,kt", [(1, 0), (16, 0), (64, 0), (70, 0), (64, 16)])
+def test_spmm_value_kinds(kind, r, k, kt):
+    """Every value kind the library multiplies with k columns: the float and
+    bfloat16 views use the 64-column scalar-record path (k = 64, 70) and the
+    narrower chunks; the product is checked against the dense product of the
+    values the matrix actually holds (normwise 1e-12 double, 1e-5 float)."""
+    rp, col, val = clustered_csr(601, 450, seed=11 * r + k, empty_frac=0.1)
+    A, vt, vheld = _matrix_of_kind(kind, rp, col, val, r, 450)
+    rng = np.random.default_rng(k + r)
+    X = rng.uniform(-1, 1, (450, k)).astype(vt)
+    Y0 = rng.uniform(-1, 1, (601, k)).astype(vt)
+    Y = dev(Y0.copy())
+    A.spmm(dev(X), Y, alpha=1.0, beta=1.0, kt=kt)
+    Ad = dense_of(rp, col, vheld, 450)
+    ref = Ad @ X.astype(np.float64) + Y0
+    scale = max(np.abs(Ad).sum(axis=1).max() * np.abs(X).max(), np.abs(Y0).max())
+    err = np.abs(Y.cpu().numpy().astype(np.float64) - ref).max() / scale
+    assert err <= (1e-12 if vt == np.float64 else 1e-5), err
+
+
+@pytest.mark.parametrize("kind", ("f64", "lowp_f32"))
+def test_spmm_single_column_paths(kind):
+    """k = 1 with a contiguous column runs the single-vector product; a
+    column taken from a wider block (ldx, ldy > 1) runs the multi-column
+    kernel.  Both equal the dense product, with alpha and beta applied."""
+    rp, col, val = clustered_csr(500, 500, seed=4, empty_frac=0.05)
+    A, vt, vheld = _matrix_of_kind(kind, rp, col, val, 4, 500)
+    rng = np.random.default_rng(9)
+    Xw = rng.uniform(-1, 1, (500, 5)).astype(vt)
+    Yw0 = rng.uniform(-1, 1, (500, 5)).astype(vt)
+    Ad = dense_of(rp, col, vheld, 500)
+    ref = 1.5 * (Ad @ Xw[:, 2].astype(np.float64)) + 0.5 * Yw0[:, 2]
+    scale = max(np.abs(Ad).sum(axis=1).max(), 1.0)
+    tol = 1e-12 if vt == np.float64 else 1e-5
+    y = dev(Yw0[:, 2].copy().reshape(500, 1))
+    A.spmm(dev(Xw[:, 2].copy().reshape(500, 1)), y, alpha=1.5, beta=0.5)
+    assert np.abs(y.cpu().numpy()[:, 0] - ref).max() / scale <= tol
+    Yw = dev(Yw0.copy())
+    A.spmm(dev(Xw)[:, 2:3], Yw[:, 2:3], alpha=1.5, beta=0.5)
+    got = Yw.cpu().numpy()
+    assert np.abs(got[:, 2] - ref).max() / scale <= tol
+    assert np.array_equal(np.delete(got, 2, axis=1), np.delete(Yw0, 2, axis=1))  # other columns untouched
+
+
+@pytest.mark.parametrize("kind", ("f64", "lowp_f32"))
+@pytest.mark.parametrize("k", (1, 4, 16, 64))
+def test_spmm_stencil27(kind, k):
+    """The multi-right-hand-side workload's matrix family at a reduced size:
+    the 27-point stencil (20^3 rows, diagonal 26.001, off-diagonal -1) with
+    r and the column chunk kt picked by the autotuner, as the benchmark
+    does, against numpy's product of the decoded stencil."""
+    import scipy.sparse as sp
+    from spc5_amd import autotune, synth
+    rp, col, val = synth.stencil27(20)
+    n = len(rp) - 1
+    r, _ = autotune.choose_r(dev(rp), dev(col), dev(val), mode="model")
+    A, vt, vheld = _matrix_of_kind(kind, rp, col, val, r, n)
+    kt, _ = autotune.tune_spmm_kt(A, k)
+    rng = np.random.default_rng(k)
+    X = rng.uniform(-1, 1, (n, k)).astype(vt)
+    Y0 = rng.uniform(-1, 1, (n, k)).astype(vt)
+    Y = dev(Y0.copy())
+    A.spmm(dev(X), Y, alpha=1.0, beta=1.0, kt=kt)
+    As = sp.csr_matrix((vheld, col, rp), shape=(n, n))
+    ref = As @ X.astype(np.float64) + Y0
+    scale = 54.001 * 1.0
+    err = np.abs(Y.cpu().numpy().astype(np.float64) - ref).max() / scale
+    assert err <= (1e-12 if vt == np.float64 else 1e-5), (r, kt, err)
+
+
 def test_spmm_strided_and_beta0():
     from spc5_amd.device import DeviceMatrix
     rp, col, val = clustered_csr(300, 300, seed=3)

@@ -308,6 +308,66 @@ def test_clustered_runs_full_size_properties(r):
     torch.cuda.empty_cache()
 
 
+BIG_CASES = [("clustered", "f64", r) for r in RS] + [("clustered", "f32", r) for r in RS] + \
+    [("stencil27", "f64", 2), ("stencil27", "f64", 4)]
+
+
+@pytest.mark.parametrize("case,tag,r", BIG_CASES)
+def test_full_size_converter_digests(case, tag, r):
+    """Full-size GPU conversion, byte for byte: the clustered-runs matrix
+    (4,194,304 rows, double beta(r,8) and float beta(r,16)) and the 27-point
+    stencil 160^3 (double beta(2,8), beta(4,8)) give the block counts and
+    the SHA-256 digests of rowsSize (block-row pointers), headers and values
+    that the reference converter gives on the same input
+    (tests/golden/big_digests.json, written by make_golden.py)."""
+    from spc5_amd import synth
+    from spc5_amd.device import DeviceMatrix
+    ref = json.load(open(os.path.join(GOLDEN, "big_digests.json")))[case]
+    rp, col, v = synth.clustered_runs(4_194_304, seed=1, value_seed=2) if case == "clustered" else synth.stencil27(160)
+    assert len(rp) - 1 == ref["n"] and int(rp[-1]) == ref["nnz"]
+    m = DeviceMatrix.from_csr(dev(rp), dev(col), dev(v.astype(DT[tag])), r)
+    want = ref[f"{tag}_r{r}"]
+    assert m.nblocks == want["nblocks"]
+    assert sha(host(m.rows_size)) == want["rowsSize_sha256"]
+    assert sha(host(m.headers)) == want["headers_sha256"]
+    assert sha(host(m.values)) == want["values_sha256"]
+    del m
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("r", RS)
+def test_clustered_runs_full_size_float(r):
+    """The clustered-runs matrix in float, the reference's float format
+    beta(r,16), at full size: mask popcount = nnz, linearity within float
+    rounding, and 2,000 sampled rows against numpy's double sums of the
+    float values, normwise 1e-5 (sum |a_ij| |x_j| per row as the scale)."""
+    from spc5_amd import synth
+    from spc5_amd.device import DeviceMatrix
+    N = 4_194_304
+    rp, col, v64 = synth.clustered_runs(N)
+    v = v64.astype(np.float32)
+    m = DeviceMatrix.from_csr(dev(rp), dev(col), dev(v), r)
+    assert m.vs == 16 and m.info()["nnz"] == len(v)
+    x1 = dev(synth.uniform(N, 3).astype(np.float32))
+    x2 = dev(synth.uniform(N, 4).astype(np.float32))
+    y1 = torch.zeros(N, dtype=torch.float32, device="cuda")
+    y2 = torch.zeros_like(y1)
+    y12 = torch.zeros_like(y1)
+    m.spmv(x1, y1)
+    m.spmv(x2, y2)
+    m.spmv(x1 + x2, y12)
+    assert float((y12 - y1 - y2).abs().max()) <= 1e-5 * 512.0 * 2.0
+    rows = np.random.default_rng(1).choice(N, 2000, replace=False)
+    hx = host(x1).astype(np.float64)
+    yh = host(y1)
+    for i in rows:
+        s, e = rp[i], rp[i + 1]
+        exact = float(np.dot(v[s:e].astype(np.float64), hx[col[s:e]]))
+        assert abs(float(yh[i]) - exact) <= 1e-5 * max(1.0, float(np.abs(v[s:e]).sum()))
+    del m
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("tag", ["f64", "f32"])
 @pytest.mark.parametrize("r", RS)
 @pytest.mark.parametrize("row_sort", [False, True])
