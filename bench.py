@@ -26,6 +26,11 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The CPU baseline's OpenMP threads are bound one per core, as the
+# reference's job scripts do (jobs-plafrim.sh:18-19); set before any library
+# loads libgomp.
+os.environ.setdefault("OMP_PROC_BIND", "true")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "SpMV GFLOP/s + HBM GB/s (% of peak); mixed vs FP64 solve time at 1e-10"
@@ -95,6 +100,32 @@ def copy_bandwidth():
     return 2 * n * 8 / (ms * 1e-3) / 1e9
 
 
+def stream_ceiling(read_bytes, write_bytes, grids=(1024, 2048, 4096, 8192), steps=20):
+    """Rate of the library's read-dominated streaming probe (non-temporal
+    16-byte loads over the whole device) with the byte mix of one product:
+    the bandwidth a kernel that only streams these bytes reaches on this
+    device.  Returns (GB/s of the best grid, {grid: GB/s})."""
+    import ctypes as C
+
+    import torch
+    from spc5_amd._lib import check, lib
+    rb = read_bytes // 16 * 16
+    wb = max(16, write_bytes // 16 * 16)
+    src = torch.empty(rb // 8, dtype=torch.float64, device="cuda").uniform_(-1, 1)
+    dst = torch.empty(wb // 8, dtype=torch.float64, device="cuda")
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rates = {}
+    for g in grids:
+        def run():
+            check(lib().spc5_hip_stream_probe(C.c_void_p(src.data_ptr()), rb, C.c_void_p(dst.data_ptr()), wb, g,
+                                              stream), "stream probe")
+        t = _events_time(run, steps, 3)
+        rates[g] = round((rb + wb) / t / 1e9, 1)
+    del src, dst
+    torch.cuda.empty_cache()
+    return max(rates.values()), rates
+
+
 def traffic_from_profiles(workload, fmt):
     p = os.path.join(ROOT, "profiles", f"pmc_spmv_{workload}_{fmt}.json")
     if not os.path.exists(p):
@@ -105,8 +136,23 @@ def traffic_from_profiles(workload, fmt):
         return None
 
 
-def cpu_baseline(rp, col, val, r, ncols, nthreads, nloops=16, max_rows=None):
-    """Reference OpenMP path (compiled reference header) on the host cores."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(rp, col, val, r, ncols, nthreads, nloops=16, nsamples=5, max_rows=None):
+    """Reference OpenMP path (compiled reference header) on the host cores:
+    one split, then `nsamples` samples of `nloops` products each (the
+    reference driver's loop, examples/load_mm_and_compare.cpp:656-729);
+    reports the median sample's rate, with the best and worst."""
+    import statistics
+
     import numpy as np
     try:
         import oracle as O
@@ -125,12 +171,18 @@ def cpu_baseline(rp, col, val, r, ncols, nthreads, nloops=16, max_rows=None):
     x = np.zeros(ncols + 15)
     x[:ncols] = synth.uniform(ncols, 3)
     y = np.zeros(n + 7)
-    th, tconv, tspmv = O.ref_omp_bench(r, rp, col, val, x, y, nloops, nthreads, ncols=ncols)
-    gflops = 2.0 * nnz * nloops / tspmv / 1e9
-    return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": th, "kind": "reference",
+    th, tconv, samples = O.ref_omp_bench(r, rp, col, val, x, y, nloops, nthreads, ncols=ncols, nsamples=nsamples)
+    rates = [2.0 * nnz * nloops / t / 1e9 for t in samples]
+    med = statistics.median(rates)
+    return {"value": round(med, 3), "unit": "GFLOP/s", "cores": th, "kind": "reference",
             "sample": f"{n} rows / {nnz} values of the same matrix, beta({r},8) FP64, "
-                      f"SPC5_Spmv_omp x{nloops} after split (conversion {tconv:.1f}s untimed)",
-            "ms_per_spmv": round(tspmv / nloops * 1e3, 3)}
+                      f"{nsamples} samples of SPC5_Spmv_omp x{nloops} after one split (conversion {tconv:.1f}s "
+                      f"untimed); value = median sample",
+            "min_gflops": round(min(rates), 3), "max_gflops": round(max(rates), 3),
+            "samples_gflops": [round(v, 3) for v in rates],
+            "ms_per_spmv": round(statistics.median(samples) / nloops * 1e3, 3),
+            "cpu_model": cpu_model(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES")}
 
 
 def solve_benchmark(n=160, rtol=1e-10):
@@ -389,6 +441,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-max-rows", type=int, default=0)
     ap.add_argument("--no-copy", action="store_true")
+    ap.add_argument("--no-ceiling", action="store_true", help="skip the streaming-probe ceiling")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 variant of the SpMV matrix")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--solve-n", type=int, default=160)
@@ -467,6 +520,16 @@ def main():
 
     bf = formats[best]
     copy_gbs = None if args.no_copy else round(copy_bandwidth(), 1)
+    ceiling = None
+    if not args.no_ceiling:
+        # the streaming probe with the product's byte mix: everything read
+        # once, y (N values) written once
+        wbytes = n * 8
+        c_gbs, c_rates = stream_ceiling(bf["bytes_per_spmv"] - wbytes, wbytes)
+        ceiling = {"gbs": c_gbs, "by_grid": {str(k): v for k, v in c_rates.items()},
+                   "read_bytes": bf["bytes_per_spmv"] - wbytes, "write_bytes": wbytes,
+                   "kernel": "k_stream_probe (non-temporal 16-byte loads, 8 in flight per thread)"}
+        log(f"[bench] streaming ceiling (read-dominated probe, product's byte mix): {c_gbs} GB/s {c_rates}")
     out = {
         "metric": METRIC,
         "value": bf["gflops"],
@@ -484,7 +547,10 @@ def main():
                    "step": "y += A x, inputs resident in HBM"},
         "roofline": {"bound": "hbm", "achieved": bf["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": bf["frac_of_peak"],
-                     "traffic": traffic_from_profiles(args.workload, best)},
+                     "traffic": traffic_from_profiles(args.workload, best),
+                     "ceiling_measured": ceiling["gbs"] if ceiling else None,
+                     "frac_of_ceiling": round(bf["gbs"] / ceiling["gbs"], 4) if ceiling else None},
+        "stream_ceiling": ceiling,
         "formats": formats,
         "formats_f32": formats_f32,
         "copy_gbs_measured": copy_gbs,
@@ -499,7 +565,7 @@ def main():
         out["batch"] = batch_benchmark(args.batch_count)
     if not args.no_cpu:
         log("[bench] CPU baseline (reference OpenMP path)...")
-        th = args.cpu_threads or os.cpu_count() or 1
+        th = args.cpu_threads or len(os.sched_getaffinity(0)) or 1
         out["cpu_baseline"] = cpu_baseline(rp, col, val, bf["r"], ncols, th,
                                            max_rows=args.cpu_max_rows or None)
     else:

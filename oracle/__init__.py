@@ -102,8 +102,8 @@ def ref():
         L.ref_thread_split_f64.argtypes = [_I, _I, _I, _I, _P, _P, _P, _I, _P, _P, _P]
         for n in ("ref_omp_bench_f64", "ref_omp_bench_f32"):
             getattr(L, n).restype = _I
-            getattr(L, n).argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _I,
-                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
+            getattr(L, n).argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _I, _I,
+                                      C.POINTER(C.c_double), _P]
         _ref = L
     return _ref
 
@@ -271,14 +271,16 @@ def ref_thread_split(r: int, rowptr, col, values, nthreads: int):
     return start, cnt, voff
 
 
-def ref_omp_bench(r: int, rowptr, col, values, x, y, nloops: int, nthreads: int, ncols=None):
-    """Reference OpenMP path: convert, split (untimed), nloops x SPC5_Spmv_omp.
+def ref_omp_bench(r: int, rowptr, col, values, x, y, nloops: int, nthreads: int, ncols=None, nsamples: int = 1):
+    """Reference OpenMP path: convert, split (untimed), then nsamples times
+    nloops x SPC5_Spmv_omp.
 
-    Returns (threads used, conversion seconds, SpMV seconds for nloops)."""
+    Returns (threads used, conversion seconds, [SpMV seconds of each sample of nloops])."""
     rowptr, col = _i32(rowptr), _i32(col)
     n = len(rowptr) - 1
-    tc, ts = C.c_double(0), C.c_double(0)
+    tc = C.c_double(0)
+    ts = np.zeros(max(1, nsamples), dtype=np.float64)
     f = ref().ref_omp_bench_f64 if values.dtype == np.float64 else ref().ref_omp_bench_f32
     th = f(r, n, n if ncols is None else ncols, len(values), _ptr(rowptr), _ptr(col), _ptr(values),
-           _ptr(x), _ptr(y), nloops, nthreads, C.byref(tc), C.byref(ts))
-    return th, tc.value, ts.value
+           _ptr(x), _ptr(y), nloops, max(1, nsamples), nthreads, C.byref(tc), _ptr(ts))
+    return th, tc.value, [float(t) for t in ts]

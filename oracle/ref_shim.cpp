@@ -102,22 +102,24 @@ int block_count(int r, int nrows, int ncols, int nnz, const int* rowptr, const i
 }
 
 // Split + OpenMP SpMV timing, as examples/load_mm_and_compare.cpp:656-729 does:
-// convert, split once (untimed), then time nloops SPC5_Spmv_omp calls.
+// convert, split once (untimed), then time nloops SPC5_Spmv_omp calls;
+// repeated nsamples times, t_spmv[s] receiving sample s's seconds.
 template <class T>
 int omp_bench(int r, int nrows, int ncols, int nnz, const int* rowptr, const int* col, const T* val,
-              const T* x, T* y, int nloops, int nthreads, double* t_convert, double* t_spmv) {
+              const T* x, T* y, int nloops, int nsamples, int nthreads, double* t_convert, double* t_spmv) {
     if (nthreads > 0) omp_set_num_threads(nthreads);
     SPC5Mat<T> m = make_csr<T>(nrows, ncols, nnz, rowptr, col, val);
     auto t0 = std::chrono::steady_clock::now();
     CSR_to_SPC5<T>(&m, fmt_of(r));
     auto t1 = std::chrono::steady_clock::now();
     std::vector<ThreadInterval<T>> iv = SPC5_split_omp<T>(m);
-    auto t2 = std::chrono::steady_clock::now();
-    for (int l = 0; l < nloops; ++l) SPC5_Spmv_omp<T>(m, x, y, iv);
-    auto t3 = std::chrono::steady_clock::now();
-    (void)t2;
+    for (int s = 0; s < nsamples; ++s) {
+        auto t2 = std::chrono::steady_clock::now();
+        for (int l = 0; l < nloops; ++l) SPC5_Spmv_omp<T>(m, x, y, iv);
+        auto t3 = std::chrono::steady_clock::now();
+        t_spmv[s] = std::chrono::duration<double>(t3 - t2).count();
+    }
     *t_convert = std::chrono::duration<double>(t1 - t0).count();
-    *t_spmv = std::chrono::duration<double>(t3 - t2).count();
     return omp_get_max_threads();
 }
 
@@ -210,13 +212,15 @@ REF_EXPORT void ref_thread_split_f64(int r, int nrows, int ncols, int nnz, const
 }
 
 REF_EXPORT int ref_omp_bench_f64(int r, int nrows, int ncols, int nnz, const int* rowptr, const int* col,
-                                 const double* val, const double* x, double* y, int nloops, int nthreads,
+                                 const double* val, const double* x, double* y, int nloops, int nsamples, int nthreads,
                                  double* t_convert, double* t_spmv) {
-    return omp_bench<double>(r, nrows, ncols, nnz, rowptr, col, val, x, y, nloops, nthreads, t_convert, t_spmv);
+    return omp_bench<double>(r, nrows, ncols, nnz, rowptr, col, val, x, y, nloops, nsamples, nthreads, t_convert,
+                          t_spmv);
 }
 
 REF_EXPORT int ref_omp_bench_f32(int r, int nrows, int ncols, int nnz, const int* rowptr, const int* col,
-                                 const float* val, const float* x, float* y, int nloops, int nthreads,
+                                 const float* val, const float* x, float* y, int nloops, int nsamples, int nthreads,
                                  double* t_convert, double* t_spmv) {
-    return omp_bench<float>(r, nrows, ncols, nnz, rowptr, col, val, x, y, nloops, nthreads, t_convert, t_spmv);
+    return omp_bench<float>(r, nrows, ncols, nnz, rowptr, col, val, x, y, nloops, nsamples, nthreads, t_convert,
+                          t_spmv);
 }
