@@ -225,8 +225,15 @@ int spc5_hip_convert_count(int vs, int r, int64_t nrows, const int* d_rowptr, co
     DeviceBuffer err;
     SPC5_HIP_TRY(err.alloc(sizeof(int)));
     SPC5_HIP_TRY(hipMemsetAsync(err.p, 0, sizeof(int), s));
+    if (nbr > 0 && d_colidx == nullptr) {
+        // A matrix without entries may come without a column array (an
+        // empty tensor's storage); the count pass then reads no column.
+        int nnz_h = 0;
+        SPC5_HIP_TRY(hipMemcpyAsync(&nnz_h, d_rowptr + nrows, sizeof(int), hipMemcpyDeviceToHost, s));
+        SPC5_HIP_TRY(hipStreamSynchronize(s));
+        SPC5_CHECK_ARG(nnz_h == 0, "colidx is null");
+    }
     if (nbr > 0) {
-        SPC5_CHECK_ARG(d_colidx != nullptr || nrows == 0, "colidx is null");
         const int key = r * 100 + vs;
         switch (key) {
         case 108: launch_count<1, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
