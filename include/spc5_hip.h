@@ -191,6 +191,19 @@ int spc5_hip_convert_fill(int value_bytes, int vs, int mask_bytes, int r, int64_
                           const int* d_rowsSize, unsigned char* d_headers_out, void* d_values_out,
                           void* stream);
 
+/* The count pass without any allocation: same arguments, results and
+ * errors as spc5_hip_convert_count.  No counterpart in the reference; it is
+ * for callers that cannot allocate (serving loops, harnesses that forbid
+ * hipMalloc).  Its flags, total and scan partials live in a fixed 64 KiB
+ * device array, one copy per GPU, held until the process ends; that limits
+ * the matrix to 16320 * 2048 - 1 (about 33.4 M) block-rows, and larger
+ * sizes are refused with SPC5_HIP_ERR_INVALID before any launch.  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (two
+ * host threads, two streams) race.  With spc5_hip_convert_fill, which
+ * allocates nothing either, a whole conversion runs allocation-free. */
+int spc5_hip_convert_count_noalloc(int vs, int r, int64_t nrows, const int* d_rowptr, const int* d_colidx,
+                                   int* d_rowsSize_out, int64_t* nblocks, void* stream);
+
 /* y = alpha * (A x) + beta * y on device vectors (y is not read when
  * beta == 0).  alpha = beta = 1 is spc5_hip_spmv.  No counterpart in the
  * reference (its kernels only accumulate); used by the solvers. */

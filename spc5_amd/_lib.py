@@ -45,6 +45,7 @@ SIGNATURES = {
     "spc5_hip_mm_read": (_I, [C.c_char_p, _I64, _P, _P, _P, C.POINTER(_I64)]),
     "spc5_hip_spmv": (_I, [_P, _P, _P, _P]),
     "spc5_hip_convert_count": (_I, [_I, _I, _I64, _P, _P, _P, C.POINTER(_I64), _P]),
+    "spc5_hip_convert_count_noalloc": (_I, [_I, _I, _I64, _P, _P, _P, C.POINTER(_I64), _P]),
     "spc5_hip_convert_fill": (_I, [_I, _I, _I, _I, _I64, _P, _P, _P, _P, _P, _P, _P]),
     "spc5_hip_host_block_count": (_I, [_I, _I, _I64, _I64, _P, _P, C.POINTER(_I64)]),
     "spc5_hip_host_convert": (_I, [_I, _I, _I, _I, _I64, _I64, _P, _P, _P, _P, _P, _I64, _P, C.POINTER(_I64)]),

@@ -13,6 +13,8 @@
 //           (conversion only permutes values inside a block-row).
 #include "spc5_common.hpp"
 
+#include <mutex>
+
 #include <climits>
 
 namespace spc5 {
@@ -183,20 +185,130 @@ int launch_fill_w(int vs, int mb, int r, int64_t nrows, int64_t nbr, const int* 
     return SPC5_HIP_ERR_INVALID;
 }
 
-int check_rowptr(const int* rowptr, int64_t nrows, hipStream_t s) {
-    DeviceBuffer err;
-    SPC5_HIP_TRY(err.alloc(sizeof(int)));
-    SPC5_HIP_TRY(hipMemsetAsync(err.p, 0, sizeof(int), s));
-    const unsigned g = (unsigned)((nrows + 1 + 255) / 256);
-    hipLaunchKernelGGL(k_check_rowptr, dim3(g), dim3(256), 0, s, rowptr, nrows, err.as<int>());
-    SPC5_HIP_TRY(hipGetLastError());
-    int h = 0;
-    SPC5_HIP_TRY(hipMemcpyAsync(&h, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    SPC5_HIP_TRY(hipStreamSynchronize(s));
-    if (h) {
-        set_error("CSR row pointer must start at 0 and be non-decreasing");
+// Scratch of one count pass: words[0] flags a bad row pointer, words[1]
+// unsorted or negative columns, words[2] receives the block total; the scan
+// of the block counts uses partials[0 .. partial_cap).
+struct CountScratch {
+    int* words = nullptr;
+    int* partials = nullptr;
+    int64_t partial_cap = 0;
+};
+
+// Fixed device scratch of spc5_hip_convert_count_noalloc, one copy per GPU
+// (a file-scope __device__ array): kScratchWords flag/total words, then the
+// scan partials.  Host code uses the address hipGetSymbolAddress returns on
+// each device, never the array's name.
+constexpr int kScratchInts = 16384;  // 64 KiB
+constexpr int kScratchWords = 64;    // keeps the partials 256-byte aligned
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) int scratch[kScratchInts];
+
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's copy of `scratch` (looked up once
+// per device, with that device current).
+int scratch_address(int** out) {
+    int dev = -1;
+    SPC5_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device index out of range for the fixed scratch table");
         return SPC5_HIP_ERR_INVALID;
     }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        SPC5_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (p == nullptr) {
+            set_error("hipGetSymbolAddress returned no address for the scratch array");
+            return SPC5_HIP_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = reinterpret_cast<int*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return SPC5_HIP_OK;
+}
+
+// Reads n flag/total words back (synchronises the stream).
+int read_words(const int* d_words, int* h, int n, hipStream_t s) {
+    SPC5_HIP_TRY(hipMemcpyAsync(h, d_words, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s));
+    SPC5_HIP_TRY(hipStreamSynchronize(s));
+    return SPC5_HIP_OK;
+}
+
+// Count pass shared by spc5_hip_convert_count and its no-allocation variant:
+// row pointer check, block count per block-row, exclusive scan into
+// d_rowsSize_out[0..nbr], total into *nblocks.  Allocates nothing itself.
+int convert_count_impl(int vs, int r, int64_t nrows, const int* d_rowptr, const int* d_colidx, int* d_rowsSize_out,
+                       int64_t* nblocks, const CountScratch& ws, hipStream_t s) {
+    *nblocks = 0;
+    const int64_t nbr = (nrows + r - 1) / r;
+    if (scan_partials_needed(nbr + 1) > ws.partial_cap) {
+        set_error("too many block-rows for the scratch space (" + std::to_string(nbr) + " > " +
+                  std::to_string(ws.partial_cap * 2048 - 1) + ")");
+        return SPC5_HIP_ERR_INVALID;
+    }
+    // flags and total start at zero: a fill kernel, not a memset node
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ws.words, (int64_t)3, 0);
+    SPC5_HIP_TRY(hipGetLastError());
+    {
+        const unsigned g = (unsigned)((nrows + 1 + 255) / 256);
+        hipLaunchKernelGGL(k_check_rowptr, dim3(g), dim3(256), 0, s, d_rowptr, nrows, ws.words);
+        SPC5_HIP_TRY(hipGetLastError());
+        int h = 0;
+        int rc = read_words(ws.words, &h, 1, s);
+        if (rc) return rc;
+        // the count pass reads columns at the row pointers: only once they are valid
+        if (h) {
+            set_error("CSR row pointer must start at 0 and be non-decreasing");
+            return SPC5_HIP_ERR_INVALID;
+        }
+    }
+    if (nbr > 0 && d_colidx == nullptr) {
+        // A matrix without entries may come without a column array (an
+        // empty tensor's storage); the count pass then reads no column.
+        int nnz_h = 0;
+        SPC5_HIP_TRY(hipMemcpyAsync(&nnz_h, d_rowptr + nrows, sizeof(int), hipMemcpyDeviceToHost, s));
+        SPC5_HIP_TRY(hipStreamSynchronize(s));
+        SPC5_CHECK_ARG(nnz_h == 0, "colidx is null");
+    }
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, d_rowsSize_out + nbr, (int64_t)1, 0);
+    SPC5_HIP_TRY(hipGetLastError());
+    if (nbr > 0) {
+        const int key = r * 100 + vs;
+        switch (key) {
+        case 108: launch_count<1, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
+        case 208: launch_count<2, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
+        case 408: launch_count<4, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
+        case 808: launch_count<8, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
+        case 116: launch_count<1, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
+        case 216: launch_count<2, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
+        case 416: launch_count<4, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
+        case 816: launch_count<8, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
+        }
+        SPC5_HIP_TRY(hipGetLastError());
+    }
+    int rc = scan_exclusive_i32_ws(d_rowsSize_out, nbr + 1, ws.words + 2, ws.partials, s);
+    if (rc) return rc;
+    int h[3] = {0, 0, 0};
+    rc = read_words(ws.words, h, 3, s);
+    if (rc) return rc;
+    if (h[1]) {
+        set_error("CSR columns must be non-negative and strictly increasing inside each row");
+        return SPC5_HIP_ERR_INVALID;
+    }
+    *nblocks = h[2];
+    return SPC5_HIP_OK;
+}
+
+int check_convert_count_args(int vs, int r, int64_t nrows, const int* d_rowptr, const int* d_rowsSize_out,
+                             const int64_t* nblocks) {
+    SPC5_CHECK_ARG(nblocks != nullptr, "nblocks is null");
+    SPC5_CHECK_ARG(r == 1 || r == 2 || r == 4 || r == 8, "r must be 1, 2, 4 or 8");
+    SPC5_CHECK_ARG(vs == 8 || vs == 16, "VS must be 8 or 16");
+    SPC5_CHECK_ARG(nrows >= 0 && nrows <= INT32_MAX, "nrows out of range");
+    SPC5_CHECK_ARG(d_rowptr != nullptr && d_rowsSize_out != nullptr, "null pointer");
+    SPC5_CHECK_ARG((const void*)d_rowptr != (const void*)d_rowsSize_out, "rowsSize_out must not alias rowptr");
     return SPC5_HIP_OK;
 }
 
@@ -210,57 +322,35 @@ extern "C" {
 
 int spc5_hip_convert_count(int vs, int r, int64_t nrows, const int* d_rowptr, const int* d_colidx,
                            int* d_rowsSize_out, int64_t* nblocks, void* stream) {
-    SPC5_CHECK_ARG(nblocks != nullptr, "nblocks is null");
-    SPC5_CHECK_ARG(r == 1 || r == 2 || r == 4 || r == 8, "r must be 1, 2, 4 or 8");
-    SPC5_CHECK_ARG(vs == 8 || vs == 16, "VS must be 8 or 16");
-    SPC5_CHECK_ARG(nrows >= 0 && nrows <= INT32_MAX, "nrows out of range");
-    SPC5_CHECK_ARG(d_rowptr != nullptr && d_rowsSize_out != nullptr, "null pointer");
-    SPC5_CHECK_ARG((const void*)d_rowptr != (const void*)d_rowsSize_out, "rowsSize_out must not alias rowptr");
-    hipStream_t s = as_stream(stream);
+    int rc = check_convert_count_args(vs, r, nrows, d_rowptr, d_rowsSize_out, nblocks);
+    if (rc) return rc;
     *nblocks = 0;
+    hipStream_t s = as_stream(stream);
     const int64_t nbr = (nrows + r - 1) / r;
-    int rc = check_rowptr(d_rowptr, nrows, s);
-    if (rc) return rc;
-    SPC5_HIP_TRY(hipMemsetAsync(d_rowsSize_out + nbr, 0, sizeof(int), s));
-    DeviceBuffer err;
-    SPC5_HIP_TRY(err.alloc(sizeof(int)));
-    SPC5_HIP_TRY(hipMemsetAsync(err.p, 0, sizeof(int), s));
-    if (nbr > 0 && d_colidx == nullptr) {
-        // A matrix without entries may come without a column array (an
-        // empty tensor's storage); the count pass then reads no column.
-        int nnz_h = 0;
-        SPC5_HIP_TRY(hipMemcpyAsync(&nnz_h, d_rowptr + nrows, sizeof(int), hipMemcpyDeviceToHost, s));
-        SPC5_HIP_TRY(hipStreamSynchronize(s));
-        SPC5_CHECK_ARG(nnz_h == 0, "colidx is null");
-    }
-    if (nbr > 0) {
-        const int key = r * 100 + vs;
-        switch (key) {
-        case 108: launch_count<1, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
-        case 208: launch_count<2, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
-        case 408: launch_count<4, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
-        case 808: launch_count<8, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
-        case 116: launch_count<1, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
-        case 216: launch_count<2, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
-        case 416: launch_count<4, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
-        case 816: launch_count<8, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, err.as<int>(), s); break;
-        }
-        SPC5_HIP_TRY(hipGetLastError());
-    }
-    DeviceBuffer total;
-    SPC5_HIP_TRY(total.alloc(sizeof(int)));
-    rc = scan_exclusive_i32(d_rowsSize_out, nbr + 1, total.as<int>(), s);
-    if (rc) return rc;
-    int h = 0, herr = 0;
-    SPC5_HIP_TRY(hipMemcpyAsync(&h, total.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    SPC5_HIP_TRY(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    CountScratch ws;
+    ws.partial_cap = scan_partials_needed(nbr + 1);
+    DeviceBuffer buf;
+    SPC5_HIP_TRY(buf.alloc(sizeof(int) * (size_t)(kScratchWords + ws.partial_cap)));
+    ws.words = buf.as<int>();
+    ws.partials = ws.words + kScratchWords;
+    rc = convert_count_impl(vs, r, nrows, d_rowptr, d_colidx, d_rowsSize_out, nblocks, ws, s);
+    // the buffer is freed on return: every launch that uses it has ended
+    // (each path out of the pass ends with a stream synchronisation)
     SPC5_HIP_TRY(hipStreamSynchronize(s));
-    if (herr) {
-        set_error("CSR columns must be non-negative and strictly increasing inside each row");
-        return SPC5_HIP_ERR_INVALID;
-    }
-    *nblocks = h;
-    return SPC5_HIP_OK;
+    return rc;
+}
+
+int spc5_hip_convert_count_noalloc(int vs, int r, int64_t nrows, const int* d_rowptr, const int* d_colidx,
+                                   int* d_rowsSize_out, int64_t* nblocks, void* stream) {
+    int rc = check_convert_count_args(vs, r, nrows, d_rowptr, d_rowsSize_out, nblocks);
+    if (rc) return rc;
+    *nblocks = 0;
+    CountScratch ws;
+    rc = scratch_address(&ws.words);
+    if (rc) return rc;
+    ws.partials = ws.words + kScratchWords;
+    ws.partial_cap = kScratchInts - kScratchWords;
+    return convert_count_impl(vs, r, nrows, d_rowptr, d_colidx, d_rowsSize_out, nblocks, ws, as_stream(stream));
 }
 
 int spc5_hip_convert_fill(int value_bytes, int vs, int mask_bytes, int r, int64_t nrows, const int* d_rowptr,

@@ -98,22 +98,45 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(int* __restrict__ d
 
 }  // namespace
 
-// In-place exclusive scan of d_data[0..n); the grand total goes to *d_total
-// (device pointer, may be null).  Totals must fit int32 (checked by callers).
+__global__ void k_fill_i32(int* __restrict__ p, int64_t n, int v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int64_t scan_partials_needed(int64_t n) { return n <= 0 ? 0 : (n + kScanChunk - 1) / kScanChunk; }
+
+// In-place exclusive scan of d_data[0..n) with caller-provided partials
+// (scan_partials_needed(n) ints); the grand total goes to *d_total (device
+// pointer, may be null).  Asynchronous: the partials are in use until the
+// stream reaches the end of the launches.  Totals must fit int32 (checked by
+// callers).
+int scan_exclusive_i32_ws(int* d_data, int64_t n, int* d_total, int* d_partial, hipStream_t s) {
+    if (n <= 0) {
+        if (d_total) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, d_total, (int64_t)1, 0);
+        SPC5_HIP_TRY(hipGetLastError());
+        return SPC5_HIP_OK;
+    }
+    const int64_t nblk = scan_partials_needed(n);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nblk), dim3(kScanThreads), 0, s, d_data, n, d_partial);
+    SPC5_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, s, d_partial, nblk, d_total);
+    SPC5_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nblk), dim3(kScanThreads), 0, s, d_data, n, d_partial);
+    SPC5_HIP_TRY(hipGetLastError());
+    return SPC5_HIP_OK;
+}
+
+// The same with partials the call allocates (and frees after a stream
+// synchronisation).
 int scan_exclusive_i32(int* d_data, int64_t n, int* d_total, hipStream_t s) {
     if (n <= 0) {
         if (d_total) SPC5_HIP_TRY(hipMemsetAsync(d_total, 0, sizeof(int), s));
         return SPC5_HIP_OK;
     }
-    const int64_t nblk = (n + kScanChunk - 1) / kScanChunk;
     DeviceBuffer partial;
-    SPC5_HIP_TRY(partial.alloc(sizeof(int) * (size_t)nblk));
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nblk), dim3(kScanThreads), 0, s, d_data, n, partial.as<int>());
-    SPC5_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, s, partial.as<int>(), nblk, d_total);
-    SPC5_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nblk), dim3(kScanThreads), 0, s, d_data, n, partial.as<int>());
-    SPC5_HIP_TRY(hipGetLastError());
+    SPC5_HIP_TRY(partial.alloc(sizeof(int) * (size_t)scan_partials_needed(n)));
+    const int rc = scan_exclusive_i32_ws(d_data, n, d_total, partial.as<int>(), s);
+    if (rc) return rc;
     // partial is freed on return: wait for the kernels that use it.
     SPC5_HIP_TRY(hipStreamSynchronize(s));
     return SPC5_HIP_OK;

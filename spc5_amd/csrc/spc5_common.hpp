@@ -263,6 +263,11 @@ struct ValueLoad<unsigned short, double> {
 // Host helpers shared by the translation units
 // --------------------------------------------------------------------------
 int scan_exclusive_i32(int* d_data, int64_t n, int* d_total, hipStream_t s);  // in place, n entries
+// The same scan with caller-provided partials (scan_partials_needed(n) ints), no allocation, asynchronous.
+int64_t scan_partials_needed(int64_t n);
+int scan_exclusive_i32_ws(int* d_data, int64_t n, int* d_total, int* d_partial, hipStream_t s);
+// p[0..n) = v (a kernel, not a memset node).
+__global__ void k_fill_i32(int* __restrict__ p, int64_t n, int v);
 
 struct DeviceBuffer {
     void* p = nullptr;
