@@ -100,11 +100,12 @@ def copy_bandwidth():
     return 2 * n * 8 / (ms * 1e-3) / 1e9
 
 
-def stream_ceiling(read_bytes, write_bytes, grids=(1024, 2048, 4096, 8192), steps=20):
-    """Rate of the library's read-dominated streaming probe (non-temporal
-    16-byte loads over the whole device) with the byte mix of one product:
-    the bandwidth a kernel that only streams these bytes reaches on this
-    device.  Returns (GB/s of the best grid, {grid: GB/s})."""
+def stream_ceiling(read_bytes, write_bytes, tiles_kib=(8, 16, 32, 64), steps=20):
+    """Rate of the library's read-dominated streaming probe (each wave reads
+    one contiguous tile with non-temporal 16-byte loads, as the product's
+    waves stream their tiles) with the byte mix of one product: the
+    bandwidth a kernel that only streams these bytes reaches on this device.
+    Returns (GB/s of the best tile size, {tile KiB: GB/s})."""
     import ctypes as C
 
     import torch
@@ -115,12 +116,12 @@ def stream_ceiling(read_bytes, write_bytes, grids=(1024, 2048, 4096, 8192), step
     dst = torch.empty(wb // 8, dtype=torch.float64, device="cuda")
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     rates = {}
-    for g in grids:
+    for kib in tiles_kib:
         def run():
-            check(lib().spc5_hip_stream_probe(C.c_void_p(src.data_ptr()), rb, C.c_void_p(dst.data_ptr()), wb, g,
-                                              stream), "stream probe")
+            check(lib().spc5_hip_stream_probe(C.c_void_p(src.data_ptr()), rb, C.c_void_p(dst.data_ptr()), wb,
+                                              kib * 1024, stream), "stream probe")
         t = _events_time(run, steps, 3)
-        rates[g] = round((rb + wb) / t / 1e9, 1)
+        rates[kib] = round((rb + wb) / t / 1e9, 1)
     del src, dst
     torch.cuda.empty_cache()
     return max(rates.values()), rates
@@ -526,9 +527,10 @@ def main():
         # once, y (N values) written once
         wbytes = n * 8
         c_gbs, c_rates = stream_ceiling(bf["bytes_per_spmv"] - wbytes, wbytes)
-        ceiling = {"gbs": c_gbs, "by_grid": {str(k): v for k, v in c_rates.items()},
+        ceiling = {"gbs": c_gbs, "by_tile_kib": {str(k): v for k, v in c_rates.items()},
                    "read_bytes": bf["bytes_per_spmv"] - wbytes, "write_bytes": wbytes,
-                   "kernel": "k_stream_probe (non-temporal 16-byte loads, 8 in flight per thread)"}
+                   "kernel": "k_stream_probe (one contiguous tile per wave, non-temporal 16-byte loads, 4 KiB "
+                             "in flight per wave)"}
         log(f"[bench] streaming ceiling (read-dominated probe, product's byte mix): {c_gbs} GB/s {c_rates}")
     out = {
         "metric": METRIC,

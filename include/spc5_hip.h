@@ -311,13 +311,15 @@ int spc5_hip_synth_banded_fill(int64_t n, int64_t bw, uint64_t thr, uint64_t see
                                double* d_val, void* stream);
 
 /* Read-dominated streaming probe (no counterpart in the reference; a
- * measurement aid): reads read_bytes of d_src with non-temporal 16-byte
- * loads over `grid` workgroups of 256 threads and writes write_bytes of
+ * measurement aid): every wave reads one contiguous tile of tile_bytes of
+ * d_src with non-temporal 16-byte loads (4 KiB in flight per wave), as a
+ * product's waves stream their tiles, then the waves write write_bytes of
  * d_dst (which must hold at least 16 bytes even when write_bytes is 0).
- * Sizes are multiples of 16.  With a write share of about 3 % it has the
- * HBM mix of one SpMV, so its rate is the ceiling such a kernel can reach. */
-int spc5_hip_stream_probe(const void* d_src, int64_t read_bytes, void* d_dst, int64_t write_bytes, int grid,
-                          void* stream);
+ * Sizes are multiples of 16, tile_bytes of 1 KiB.  With a write share of
+ * about 3 % it has the HBM mix of one SpMV, so its rate is the ceiling
+ * such a kernel can reach. */
+int spc5_hip_stream_probe(const void* d_src, int64_t read_bytes, void* d_dst, int64_t write_bytes,
+                          int64_t tile_bytes, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* 3. Host-pointer helpers (used by include/spc5_hip.hpp)                 */

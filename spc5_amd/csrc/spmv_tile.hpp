@@ -272,20 +272,12 @@ __device__ __forceinline__ void row_displacements(int* segs, const unsigned long
 template <int R, int VS, int NW, int kScale = 1>
 __device__ __forceinline__ void expand_rows_sorted(int2* cp, unsigned char* row, int rbase, int cs,
                                                    const unsigned long long* mw, int p, const int* d) {
-    if constexpr (R * VS <= 16) {
-        const unsigned m = (unsigned)mw[0];
-#pragma unroll
-        for (int bit = 0; bit < R * VS; ++bit) {
-            if ((m >> bit) & 1u) {
-                const int s = p + d[bit / VS];
-                cp[p] = make_int2(cs + (bit % VS) * kScale, s);
-                row[s] = (unsigned char)(rbase + bit / VS);
-                ++p;
-            }
-        }
-    } else {
-        // One loop per block row: the row and its displacement are
-        // compile-time indices (no dynamically indexed array).
+    // One loop per block row over its set bits only: the row and its
+    // displacement are compile-time indices (no dynamically indexed array),
+    // and a loop runs as often as the wave's largest count of that row (a
+    // loop over every bit position measured 4-5 % slower for float and
+    // bfloat16 values on the 27-point stencil, whose rows hold 3 of 8).
+    {
 #pragma unroll
         for (int t = 0; t < R; ++t) {
             const int bit0 = t * VS;

@@ -55,33 +55,36 @@ __global__ void k_banded_fill(int64_t n, int64_t bw, unsigned long long thr, uns
 }
 
 
-// Read-dominated streaming probe: every thread reads 16-byte words of src
-// with non-temporal loads, U requests in flight, the grid striding over the
-// whole array (each wave reads 1 KiB contiguous per request), then writes
-// its share of dst.  With a write share of a few percent this is the HBM
-// mix of a product y = A x (about 97 % reads), so its rate is the ceiling
-// such a kernel can reach on the device.
+// Read-dominated streaming probe: each wave reads one contiguous tile of
+// `tile` 16-byte words with non-temporal loads, 4 KiB per step (four
+// 16-byte loads per lane in flight), the way a product's waves stream their
+// tiles; then the waves write dst (m16 words) in a grid-stride loop.  With
+// a write share of a few percent this is the HBM mix of a product y = A x
+// (about 97 % reads), so its rate is the ceiling such a kernel can reach.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-template <int U>
-__global__ __launch_bounds__(256) void k_stream_probe(const u32x4* __restrict__ src, int64_t n16,
+__global__ __launch_bounds__(256) void k_stream_probe(const u32x4* __restrict__ src, int64_t n16, int64_t tile,
                                                       u32x4* __restrict__ dst, int64_t m16) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t t0 = wave * tile;
+    const int64_t t1 = t0 + tile < n16 ? t0 + tile : n16;
     unsigned acc = 0;
-    int64_t i = tid;
-    for (; i + (U - 1) * stride < n16; i += U * stride) {
-        u32x4 v[U];
+    int64_t i = t0 + lane;
+    for (; i + 192 < t1; i += 256) {
+        u32x4 v[4];
 #pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(src + i + u * stride);
+        for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(src + i + 64 * u);
 #pragma unroll
-        for (int u = 0; u < U; ++u) acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+        for (int u = 0; u < 4; ++u) acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
     }
-    for (; i < n16; i += stride) {
+    for (; i < t1; i += 64) {
         const u32x4 v = __builtin_nontemporal_load(src + i);
         acc ^= v.x ^ v.y ^ v.z ^ v.w;
     }
-    for (int64_t j = tid; j < m16; j += stride) dst[j] = u32x4{acc, (unsigned)j, 0u, 0u};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m16; j += stride)
+        dst[j] = u32x4{acc, (unsigned)j, 0u, 0u};
     // The loads must not be dropped: the sum reaches memory on one
     // impossible-looking value (dst holds at least one word).
     if (m16 == 0 && acc == 0x9e3779b9u) dst[0] = u32x4{acc, 1u, 0u, 0u};
@@ -121,14 +124,18 @@ int spc5_hip_synth_banded_fill(int64_t n, int64_t bw, uint64_t thr, uint64_t see
     return SPC5_HIP_OK;
 }
 
-int spc5_hip_stream_probe(const void* d_src, int64_t read_bytes, void* d_dst, int64_t write_bytes, int grid,
-                          void* stream) {
+int spc5_hip_stream_probe(const void* d_src, int64_t read_bytes, void* d_dst, int64_t write_bytes,
+                          int64_t tile_bytes, void* stream) {
     SPC5_CHECK_ARG(d_src != nullptr && d_dst != nullptr, "null buffer");
     SPC5_CHECK_ARG(read_bytes >= 0 && write_bytes >= 0 && read_bytes % 16 == 0 && write_bytes % 16 == 0,
                    "sizes must be non-negative multiples of 16 bytes");
-    SPC5_CHECK_ARG(grid > 0 && grid <= (1 << 20), "grid must be in [1, 2^20]");
-    hipLaunchKernelGGL(k_stream_probe<8>, dim3((unsigned)grid), dim3(256), 0, as_stream(stream),
-                       reinterpret_cast<const u32x4*>(d_src), read_bytes / 16, reinterpret_cast<u32x4*>(d_dst),
+    SPC5_CHECK_ARG(tile_bytes >= 1024 && tile_bytes % 1024 == 0, "tile_bytes must be a positive multiple of 1 KiB");
+    const int64_t tile = tile_bytes / 16;
+    const int64_t waves = (read_bytes / 16 + tile - 1) / tile;
+    const int64_t groups = waves > 0 ? (waves + 3) / 4 : 1;
+    SPC5_CHECK_ARG(groups <= (1ll << 30), "too many tiles");
+    hipLaunchKernelGGL(k_stream_probe, dim3((unsigned)groups), dim3(256), 0, as_stream(stream),
+                       reinterpret_cast<const u32x4*>(d_src), read_bytes / 16, tile, reinterpret_cast<u32x4*>(d_dst),
                        write_bytes / 16);
     SPC5_HIP_TRY(hipGetLastError());
     return SPC5_HIP_OK;
