@@ -326,10 +326,12 @@ def _events_time(fn, steps, warmup):
     return e0.elapsed_time(e1) / steps * 1e-3
 
 
-def spmm_benchmark(n=160, ks=(1, 4, 16, 64), steps=10, warmup=2):
+def spmm_benchmark(n=160, ks=(1, 4, 16, 64), steps=10, warmup=2, rs=(2, 4, 8)):
     """Multi-right-hand-side products on the 27-point stencil n^3, double and
-    float values (beta(r,8)), row-major X and Y with k columns; the column
-    chunk kt is picked by the autotuner (timed candidates)."""
+    float values (beta(r,8)), row-major X and Y with k columns.  The
+    autotuner picks r, the column chunk kt and the record kind (one X row
+    request per stored value, or per set column of a block) by timing every
+    candidate; the case's time is then measured again on its own."""
     import torch
     from spc5_amd import autotune, synth
     from spc5_amd.device import DeviceMatrix
@@ -339,28 +341,35 @@ def spmm_benchmark(n=160, ks=(1, 4, 16, 64), steps=10, warmup=2):
     N = len(rp) - 1
     nnz = len(val)
     d_rp, d_col, d_val = (torch.from_numpy(a).cuda() for a in (rp, col, val))
-    r, det = autotune.choose_r(d_rp, d_col, d_val, mode="model")
-    A64 = DeviceMatrix.from_csr(d_rp, d_col, d_val, r)
+    mats64 = {r: DeviceMatrix.from_csr(d_rp, d_col, d_val, r) for r in rs}
     del d_rp, d_col, d_val
-    out = {"workload": f"stencil27 {n}^3 (N={N}, nnz={nnz}), beta({r},8) chosen by the autotuner",
-           "blocks": {str(k): v for k, v in det["blocks"].items()}, "cases": []}
+    out = {"workload": f"stencil27 {n}^3 (N={N}, nnz={nnz}), beta(r,8) for r in {list(rs)}, schedule chosen by the "
+                       f"autotuner", "blocks": {str(r): m.nblocks for r, m in mats64.items()}, "cases": []}
     g = torch.Generator(device="cuda").manual_seed(3)
-    for tag, A in (("f64", A64), ("f32", lowp(A64, "f32"))):
+    for tag in ("f64", "f32"):
+        mats = mats64 if tag == "f64" else {r: lowp(m, "f32") for r, m in mats64.items()}
         for k in ks:
-            X = torch.empty((N, k), dtype=A.vec_dtype, device="cuda").uniform_(-1, 1, generator=g)
-            Y = torch.zeros((N, k), dtype=A.vec_dtype, device="cuda")
-            kt, cand = autotune.tune_spmm_kt(A, k)
+            vt = next(iter(mats.values())).vec_dtype
+            X = torch.empty((N, k), dtype=vt, device="cuda").uniform_(-1, 1, generator=g)
+            Y = torch.zeros((N, k), dtype=vt, device="cuda")
+            (r, kt, cols), cand = autotune.tune_spmm(mats, k)
+            A = mats[r]
+            A.spmm_columns = cols
             t = _events_time(lambda: A.spmm(X, Y, kt=kt), steps, warmup)
             a_bytes = A.algorithmic_bytes() - (A.ncols + 2 * A.nrows) * A.vec_dtype.itemsize
             B = a_bytes + k * (A.ncols + 2 * A.nrows) * A.vec_dtype.itemsize
-            case = {"dtype": tag, "k": k, "kt": kt, "ms": round(t * 1e3, 4),
-                    "gflops": round(2.0 * nnz * k / t / 1e9, 1), "gbs": round(B / t / 1e9, 1),
-                    "frac_of_peak": round(B / t / 1e9 / HBM_PEAK_GBS, 4), "bytes": B,
-                    "kt_candidates_ms": {str(c): round(v * 1e3, 4) for c, v in cand.items()}}
+            case = {"dtype": tag, "k": k, "r": r, "kt": kt, "records": "per block column" if cols else "per value",
+                    "ms": round(t * 1e3, 4), "gflops": round(2.0 * nnz * k / t / 1e9, 1),
+                    "gbs": round(B / t / 1e9, 1), "frac_of_peak": round(B / t / 1e9 / HBM_PEAK_GBS, 4), "bytes": B,
+                    "candidates_ms": {f"r={cr} kt={ckt} {'columns' if cc else 'values'}": round(v * 1e3, 4)
+                                      for (cr, ckt, cc), v in sorted(cand.items())}}
             out["cases"].append(case)
-            log(f"[bench] SpMM {tag} k={k} kt={kt}: {t * 1e3:.3f} ms {case['gflops']} GFLOP/s {case['gbs']} GB/s")
+            log(f"[bench] SpMM {tag} k={k} beta({r},8) kt={kt} {case['records']}: {t * 1e3:.3f} ms "
+                f"{case['gflops']} GFLOP/s {case['gbs']} GB/s")
             del X, Y
-    del A64
+        if tag == "f32":
+            del mats
+    del mats64
     torch.cuda.empty_cache()
     return out
 

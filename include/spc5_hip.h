@@ -146,6 +146,12 @@ int spc5_hip_mat_load(const char* path, spc5_hip_mat_t* out);
  * for irregular masks.  Results agree with the default order to rounding.
  * No counterpart in the reference (a property of the GPU kernel). */
 #define SPC5_HIP_OPT_ROW_SORT 1
+/* SPMM_COLUMNS (0 or 1, default 0; used for r = 2, 4 and 8): spc5_hip_spmm
+ * requests each X row once per set column of a block and applies it to the
+ * block's r rows, instead of once per stored value.  Pays off where the
+ * rows of a block share columns (stencils); results agree with the default
+ * to rounding.  No counterpart in the reference. */
+#define SPC5_HIP_OPT_SPMM_COLUMNS 2
 int spc5_hip_mat_set_option(spc5_hip_mat_t h, int option, int64_t value);
 
 /* y += A x on device pointers (x and y in the matrix's vector type).

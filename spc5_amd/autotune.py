@@ -22,7 +22,8 @@ not used here.
 
 `mode="measure"` converts the matrix with each candidate r and times it
 instead (exact, costs four conversions); `tune_spmm_kt` times the candidate
-column chunks of a product with k right-hand sides.
+column chunks of a product with k right-hand sides, and `tune_spmm` picks r,
+the chunk and the record kind (per value or per block column) by timing.
 """
 from __future__ import annotations
 
@@ -172,6 +173,28 @@ def tune_spmm_kt(m: DeviceMatrix, k: int, candidates=None, reps=5):
         res[kt] = e0.elapsed_time(e1) / reps * 1e-3
     best = min(res, key=res.get)
     return best, res
+
+
+def tune_spmm(mats, k: int, kt_candidates=None, reps=5):
+    """Fastest schedule for Y += A X with k right-hand sides over candidate
+    block shapes of one matrix: `mats` maps r -> DeviceMatrix (the same
+    matrix converted with that r).  Tries every r, column chunk kt and
+    record kind (one record per stored value, or per set column of a block
+    with r > 1, SPC5_HIP_OPT_SPMM_COLUMNS) and leaves each matrix set to its
+    best record kind.  Returns ((r, kt, columns), {(r, kt, columns): seconds})."""
+    times = {}
+    for r, m in mats.items():
+        best_cols, best_t = False, None
+        for cols in ((False, True) if r > 1 else (False,)):
+            m.spmm_columns = cols
+            _, res = tune_spmm_kt(m, k, kt_candidates, reps)
+            for kt, t in res.items():
+                times[(r, kt, cols)] = t
+                if best_t is None or t < best_t:
+                    best_t, best_cols = t, cols
+        m.spmm_columns = best_cols
+    best = min(times, key=times.get)
+    return best, times
 
 
 def calibration_set():

@@ -84,6 +84,7 @@ struct spc5_hip_mat {
     const unsigned char* hdr = nullptr;   // device, [nblocks*(4+r*mb)]
     const void* values = nullptr;         // device, [nnz]
     int row_sort = 0;                     // SPC5_HIP_OPT_ROW_SORT
+    int spmm_cols = 0;                    // SPC5_HIP_OPT_SPMM_COLUMNS
     spc5::Plan plan;
     std::vector<std::shared_ptr<spc5::DeviceBuffer>> owned;  // buffers the handle allocated itself
 };

@@ -303,6 +303,281 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
     }
 }
 
+// ---- column records ------------------------------------------------------
+//
+// In a beta(r,VS) block the r rows share one window of VS columns, and one
+// column is usually set in several rows (r = 4 on the 27-point stencil: 12
+// values over 6 columns).  This kernel makes one record per (block, set
+// column): the X row offset, the block-row and the r values of that column
+// (0 where a row has none).  Each record requests its X row once and adds
+// v[t] * x into r running sums, one per row of the block-row, written to
+// the tile's LDS sums when the block-row changes.  X is then requested
+// once per distinct column of a block instead of once per value.
+
+template <class ACC, int R>
+struct alignas(16) SpmmColRec {
+    int col;   // X row: byte offset from X, or the column (WIDE)
+    int brow;  // tile-local block-row
+    ACC v[R];  // the column's value in each row of the block (0 where none)
+};
+
+// Records and values per batch (LDS about 4-5 KiB of records per wave).
+template <int R>
+constexpr int spmm_col_cap_rec() { return R == 2 ? 128 : (R == 4 ? 96 : 64); }
+template <int R>
+constexpr int spmm_col_cap_val() { return R == 2 ? 256 : (R == 4 ? 384 : 512); }
+
+// With 64-column chunks one wave walks the whole record sequence of its
+// tile in block-row order, so it writes each finished block-row of Y
+// directly (no LDS row x column sums, tiles of up to kMaxRows rows).
+template <int KC>
+constexpr bool spmm_col_direct() { return KC == 64; }
+
+template <class ACC, int R>
+size_t spmm_col_smem_bytes(int maxrows, int kt, bool direct) {
+    return sizeof(SpmmColRec<ACC, R>) * spmm_col_cap_rec<R>() + (direct ? 0 : sizeof(ACC) * (size_t)maxrows * kt) +
+           sizeof(int) * (maxrows / R + 1) + sizeof(unsigned short) * spmm_col_cap_val<R>();
+}
+
+// Columns set in any row of a block (the OR of its r row masks).
+template <int R, int VS, int NW>
+__device__ __forceinline__ unsigned block_columns(const unsigned long long* mw) {
+    unsigned cm = 0;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const int bit0 = t * VS;
+        cm |= (unsigned)((mw[bit0 / 64] >> (bit0 % 64)) & ((1ull << VS) - 1ull));
+    }
+    return cm;
+}
+
+template <class V, class X, class ACC, int R, int VS, int MB, int KC, bool WIDE>
+__global__ __launch_bounds__(64) void k_spmm_cols(const int* __restrict__ bptr, const unsigned char* __restrict__ hdr,
+                                                   const V* __restrict__ val, const X* __restrict__ x, int64_t ldx,
+                                                   X* __restrict__ y, int64_t ldy, int k, int kt,
+                                                   const int* __restrict__ tile_br, const int* __restrict__ tile_val,
+                                                   int64_t ntiles, int64_t nrows, int maxrows, ACC alpha, ACC beta) {
+    constexpr int CAPR = spmm_col_cap_rec<R>();
+    constexpr int CAP = spmm_col_cap_val<R>();
+    constexpr int K = CAP / 64;
+    constexpr int NW = MaskWords<R * MB>::kWords;
+    constexpr int G = 64 / KC;           // record groups per wave
+    constexpr int U = R <= 2 ? 16 : 8;  // X rows requested together
+    constexpr bool DIRECT = spmm_col_direct<KC>();
+    using Rec = SpmmColRec<ACC, R>;
+    static_assert(sizeof(Rec) % 16 == 0, "16-byte aligned records");
+    extern __shared__ __align__(16) unsigned char smem[];
+    Rec* rec = reinterpret_cast<Rec*>(smem);                                          // [CAPR]
+    ACC* acc = reinterpret_cast<ACC*>(rec + CAPR);                                    // [maxrows * kt] unless DIRECT
+    int* rs = reinterpret_cast<int*>(acc + (DIRECT ? 0 : (size_t)maxrows * kt));      // [maxrows / R + 1]
+    unsigned short* slot = reinterpret_cast<unsigned short*>(rs + (maxrows / R + 1));  // [CAP]
+
+    const int64_t t = blockIdx.x;
+    if (t >= ntiles) return;
+    const int c0 = blockIdx.y * kt;
+    const int ktc = (k - c0) < kt ? (k - c0) : kt;
+    const int lane = lane_id();
+    const int grp = lane / KC;
+    const int jl = lane % KC;
+    const bool colok = jl < ktc;
+    const X* __restrict__ xcol = x + c0 + (colok ? jl : 0);
+    const unsigned lane_off = (unsigned)(c0 + (colok ? jl : 0)) * (unsigned)sizeof(X);
+    const unsigned row_bytes = (unsigned)ldx * (unsigned)sizeof(X);
+    const int br0 = tile_br[t], br1 = tile_br[t + 1];
+    const int nbr = br1 - br0;
+    const int b0 = bptr[br0], b1 = bptr[br1];
+    int v = tile_val[t];
+    const int64_t row0 = (int64_t)br0 * R;
+    const int64_t rows_left = nrows - row0;
+    const int nrow_t = (int)((int64_t)nbr * R < rows_left ? (int64_t)nbr * R : rows_left);
+
+    for (int i = lane; i <= nbr; i += 64) rs[i] = bptr[br0 + i] - b0;
+    if constexpr (!DIRECT) {
+        for (int i = lane; i < nbr * R * kt; i += 64) acc[i] = (ACC)0;
+    }
+    int top = 1;
+    while (top < nbr) top <<= 1;
+
+    // Running sums of the current block-row, one per row of it; with DIRECT
+    // they persist across batches (the wave walks the tile's records in
+    // block-row order) and finished block-rows go straight to Y.
+    ACC run[R];
+#pragma unroll
+    for (int tr = 0; tr < R; ++tr) run[tr] = (ACC)0;
+    int pb = -1;    // block-row the running sums belong to
+    int done = 0;   // DIRECT: block-rows [0, done) of the tile are written
+    // DIRECT: Y of block-row pb, requested when the block-row opens so that
+    // the read is in flight while its records are processed
+    X yv[DIRECT ? R : 1];
+    auto request_y = [&](int brow) {
+        if constexpr (DIRECT) {
+#pragma unroll
+            for (int tr = 0; tr < R; ++tr) {
+                const int rr = brow * R + tr;
+                yv[tr] = (beta != (ACC)0 && brow >= 0 && rr < nrow_t && colok) ? y[(row0 + rr) * ldy + c0 + jl]
+                                                                                 : (X)0;
+            }
+        }
+    };
+    auto write_blockrow = [&](int brow, bool have) {
+#pragma unroll
+        for (int tr = 0; tr < R; ++tr) {
+            const int rr = brow * R + tr;
+            if (rr < nrow_t && colok) {
+                X* yp = &y[(row0 + rr) * ldy + c0 + jl];
+                const ACC s = have ? run[tr] : (ACC)0;
+                const ACC yo = beta == (ACC)0 ? (ACC)0 : (have ? (ACC)yv[tr] : (ACC)*yp);
+                *yp = (X)(alpha * s + beta * yo);
+            }
+        }
+    };
+    // block-rows [done, upto) hold no value: y = beta * y (nothing when beta == 1)
+    auto skip_to = [&](int upto) {
+        if (beta != (ACC)1) {
+            for (int br = done; br < upto; ++br) write_blockrow(br, false);
+        }
+        done = upto > done ? upto : done;
+    };
+    auto flush = [&](int brow) {
+        if (pb >= 0) {
+            if constexpr (DIRECT) {
+                skip_to(pb);
+                write_blockrow(pb, true);
+                done = pb + 1;
+            } else if (colok) {
+#pragma unroll
+                for (int tr = 0; tr < R; ++tr) atomicAdd(&acc[(pb * R + tr) * kt + jl], run[tr]);
+            }
+        }
+#pragma unroll
+        for (int tr = 0; tr < R; ++tr) run[tr] = (ACC)0;
+        pb = brow;
+        request_y(brow);
+    };
+
+    int b = b0;
+    int cs = 0;
+    unsigned long long mw[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) mw[w] = 0ull;
+    if (b0 < b1) load_header_clamped<R, MB>(hdr, b + lane, b1 - 1, cs, mw);
+    wave_sync();
+    while (b < b1) {
+        const bool valid = b + lane < b1;
+        unsigned long long mw_cur[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) mw_cur[w] = valid ? mw[w] : 0ull;
+        int cs_cur = valid ? cs : 0;
+        asm volatile("" : "+v"(cs_cur));
+        int cnt = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) cnt += __popcll(mw_cur[w]);
+        const unsigned cm = block_columns<R, VS, NW>(mw_cur);
+        const int crec = __popc(cm);
+        const int incl = wave_incl_scan(cnt);
+        const int inclr = wave_incl_scan(crec);
+        const int ntake = __popcll(ballot64(valid && incl <= CAP && inclr <= CAPR));
+        const int total = __builtin_amdgcn_readlane(incl, ntake - 1);
+        const int nrec = __builtin_amdgcn_readlane(inclr, ntake - 1);
+        ACC a[K];
+        load_values<V, ACC, K>(val, v, total, lane, a);
+        const int jcur = b + lane;
+        const int bn = b + ntake;
+        load_header_clamped<R, MB>(hdr, bn + lane, b1 - 1, cs, mw);
+        // records of this batch start at zero (every lane clears 16-byte words)
+        {
+            uint4* words = reinterpret_cast<uint4*>(rec);
+            const int nw = nrec * (int)(sizeof(Rec) / 16);
+            for (int i = lane; i < nw; i += 64) words[i] = make_uint4(0u, 0u, 0u, 0u);
+        }
+        wave_sync();
+        if (lane < ntake) {
+            // record header per set column, and the record slot of every value
+            const int brow = find_blockrow(rs, nbr, top, jcur - b0);
+            const unsigned cbase = WIDE ? (unsigned)cs_cur : (unsigned)cs_cur * row_bytes;
+            const int rb = inclr - crec;
+            unsigned m = cm;
+            int i = rb;
+            while (m) {
+                const unsigned c = (unsigned)__builtin_ctz(m);
+                rec[i].col = (int)(WIDE ? cbase + c : cbase + __umul24(c, row_bytes));
+                rec[i].brow = brow;
+                ++i;
+                m &= m - 1u;
+            }
+            int p = incl - cnt;
+#pragma unroll
+            for (int tr = 0; tr < R; ++tr) {
+                const int bit0 = tr * VS;
+                unsigned mr = (unsigned)((mw_cur[bit0 / 64] >> (bit0 % 64)) & ((1ull << VS) - 1ull));
+                while (mr) {
+                    const unsigned c = (unsigned)__builtin_ctz(mr);
+                    const int ri = rb + __popc(cm & ((1u << c) - 1u));  // record of column c
+                    slot[p] = (unsigned short)(ri * (int)sizeof(Rec) + 8 + tr * (int)sizeof(ACC));
+                    ++p;
+                    mr &= mr - 1u;
+                }
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            const int q = lane + 64 * kk;
+            if (q < total) *reinterpret_cast<ACC*>(reinterpret_cast<unsigned char*>(rec) + slot[q]) = a[kk];
+        }
+        wave_sync();
+        // group grp walks records [q, qb)
+        const int per = (nrec + G - 1) / G;
+        int q = grp * per;
+        const int qb = q + per < nrec ? q + per : nrec;
+        for (; q + U <= qb; q += U) {
+            int cu[U], bu[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int2 hb = *reinterpret_cast<const int2*>(&rec[q + u]);
+                cu[u] = hb.x;
+                bu[u] = hb.y;
+            }
+            // all U X rows are requested before the first use waits
+            __builtin_amdgcn_sched_barrier(0);
+            ACC xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = (ACC)spmm_x<WIDE>(xcol, x, cu[u], ldx, lane_off);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (bu[u] != pb) flush(bu[u]);
+#pragma unroll
+                for (int tr = 0; tr < R; ++tr) run[tr] += rec[q + u].v[tr] * xv[u];
+            }
+        }
+        for (; q < qb; ++q) {
+            const int2 hb = *reinterpret_cast<const int2*>(&rec[q]);
+            const ACC xv = (ACC)spmm_x<WIDE>(xcol, x, hb.x, ldx, lane_off);
+            if (hb.y != pb) flush(hb.y);
+#pragma unroll
+            for (int tr = 0; tr < R; ++tr) run[tr] += rec[q].v[tr] * xv;
+        }
+        // groups end their runs with the batch (each batch splits anew);
+        // one group keeps its block-row open across batches
+        if constexpr (!DIRECT) flush(-1);
+        v += total;
+        b = bn;
+        wave_sync();
+    }
+    if constexpr (DIRECT) {
+        flush(-1);
+        skip_to(nbr);
+    } else {
+        wave_sync();
+        for (int i = lane; i < nrow_t * ktc; i += 64) {
+            const int rr = i / ktc, j = i % ktc;
+            X* yp = &y[(row0 + rr) * ldy + c0 + j];
+            const ACC s = acc[rr * kt + j];
+            *yp = beta == (ACC)0 ? (X)(alpha * s) : (X)(alpha * s + beta * (ACC)*yp);
+        }
+    }
+}
+
 struct SpmmPlans {
     std::mutex mu;
     std::map<const spc5_hip_mat*, std::map<int, Plan>> plans;
@@ -313,8 +588,36 @@ SpmmPlans& spmm_plans() {
 }
 
 template <class V, class X, class ACC, int VS, int MB, int KC, bool WIDE>
+int launch_spmm_cols(const spc5_hip_mat& m, const Plan& p, int maxrows, const void* x, int64_t ldx, void* y,
+                     int64_t ldy, int k, int kt, double alpha, double beta, hipStream_t s) {
+    const dim3 grid((unsigned)p.ntiles, (unsigned)((k + kt - 1) / kt));
+    const int* tb = p.tile_br->as<int>();
+    const int* tv = p.tile_val->as<int>();
+    const V* val = reinterpret_cast<const V*>(m.values);
+    const X* xx = reinterpret_cast<const X*>(x);
+    X* yy = reinterpret_cast<X*>(y);
+#define SPC5_SPMM_COLS(R)                                                                                          \
+    {                                                                                                              \
+        const size_t shm = spmm_col_smem_bytes<ACC, R>(maxrows, kt, spmm_col_direct<KC>());                       \
+        hipLaunchKernelGGL((k_spmm_cols<V, X, ACC, R, VS, MB, KC, WIDE>), grid, dim3(64), shm, s, m.bptr, m.hdr, val, \
+                           xx, ldx, yy, ldy, k, kt, tb, tv, p.ntiles, m.nrows, maxrows, (ACC)alpha, (ACC)beta);       \
+    }
+    switch (m.r) {
+    case 2: SPC5_SPMM_COLS(2); break;
+    case 4: SPC5_SPMM_COLS(4); break;
+    case 8: SPC5_SPMM_COLS(8); break;
+    default: set_error("column records need r = 2, 4 or 8"); return SPC5_HIP_ERR_INVALID;
+    }
+#undef SPC5_SPMM_COLS
+    SPC5_HIP_TRY(hipGetLastError());
+    return SPC5_HIP_OK;
+}
+
+template <class V, class X, class ACC, int VS, int MB, int KC, bool WIDE>
 int launch_spmm_kc(const spc5_hip_mat& m, const Plan& p, int maxrows, const void* x, int64_t ldx, void* y,
                    int64_t ldy, int k, int kt, double alpha, double beta, hipStream_t s) {
+    if (m.spmm_cols && m.r > 1)
+        return launch_spmm_cols<V, X, ACC, VS, MB, KC, WIDE>(m, p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s);
     const dim3 grid((unsigned)p.ntiles, (unsigned)((k + kt - 1) / kt));
     const size_t shm = spmm_smem_bytes<ACC, KC, X>(maxrows, kt, m.r);
     const int* tb = p.tile_br->as<int>();
@@ -339,7 +642,11 @@ int launch_spmm_kc(const spc5_hip_mat& m, const Plan& p, int maxrows, const void
 template <class V, class X, class ACC, int VS, int MB>
 int launch_spmm_t(const spc5_hip_mat& m, const void* x, int64_t ldx, void* y, int64_t ldy, int k, int kt,
                   double alpha, double beta, hipStream_t s) {
-    const int maxrows = spmm_max_rows<ACC>(kt);
+    int kc = 1;  // lanes per record: the smallest power of two holding the chunk's kt columns
+    while (kc < kt) kc <<= 1;
+    // column records with 64-column chunks write Y directly: whole SpMV-sized tiles
+    const bool direct = m.spmm_cols && m.r > 1 && kc == 64;
+    const int maxrows = direct ? kMaxRows : spmm_max_rows<ACC>(kt);
     Plan* p = nullptr;
     {
         SpmmPlans& sp = spmm_plans();
@@ -358,8 +665,6 @@ int launch_spmm_t(const spc5_hip_mat& m, const void* x, int64_t ldx, void* y, in
         p = &it->second;
     }
     if (p->ntiles == 0) return SPC5_HIP_OK;
-    int kc = 1;  // lanes per record: the smallest power of two holding the chunk's kt columns
-    while (kc < kt) kc <<= 1;
     // 32-bit X offsets when every X row offset plus a column fits, and a
     // row is under 16 MiB (the 24-bit multiply of the expansion)
     const uint64_t row_bytes = (uint64_t)ldx * sizeof(X);

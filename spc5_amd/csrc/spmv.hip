@@ -378,6 +378,10 @@ int spc5_hip_mat_set_option(spc5_hip_mat_t h, int option, int64_t value) {
         SPC5_CHECK_ARG(value == 0 || value == 1, "ROW_SORT must be 0 or 1");
         h->row_sort = (int)value;
         return SPC5_HIP_OK;
+    case SPC5_HIP_OPT_SPMM_COLUMNS:
+        SPC5_CHECK_ARG(value == 0 || value == 1, "SPMM_COLUMNS must be 0 or 1");
+        h->spmm_cols = (int)value;
+        return SPC5_HIP_OK;
     default:
         set_error("unknown option");
         return SPC5_HIP_ERR_INVALID;

@@ -12,7 +12,8 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._lib import SPC5_HIP_BF16, SPC5_HIP_F32, SPC5_HIP_F64, SPC5_HIP_OPT_ROW_SORT, Spc5Error, check, lib
+from ._lib import (SPC5_HIP_BF16, SPC5_HIP_F32, SPC5_HIP_F64, SPC5_HIP_OPT_ROW_SORT, SPC5_HIP_OPT_SPMM_COLUMNS,
+                   Spc5Error, check, lib)
 
 _VT = {torch.float64: SPC5_HIP_F64, torch.float32: SPC5_HIP_F32, torch.bfloat16: SPC5_HIP_BF16}
 _VEC_DTYPE = {SPC5_HIP_F64: torch.float64, SPC5_HIP_F32: torch.float32, SPC5_HIP_BF16: torch.float32}
@@ -182,6 +183,18 @@ class DeviceMatrix:
     def row_sort(self, on: bool):
         check(lib().spc5_hip_mat_set_option(self._h, SPC5_HIP_OPT_ROW_SORT, 1 if on else 0), "set_option")
         self._row_sort = bool(on)
+
+    @property
+    def spmm_columns(self) -> bool:
+        """Multi-right-hand-side products request each X row once per set
+        column of a block (r > 1; see SPC5_HIP_OPT_SPMM_COLUMNS in
+        include/spc5_hip.h)."""
+        return getattr(self, "_spmm_columns", False)
+
+    @spmm_columns.setter
+    def spmm_columns(self, on: bool):
+        check(lib().spc5_hip_mat_set_option(self._h, SPC5_HIP_OPT_SPMM_COLUMNS, 1 if on else 0), "set_option")
+        self._spmm_columns = bool(on)
 
     @property
     def format_name(self):

@@ -185,3 +185,62 @@ def test_spmm_wide_x_rows():
     assert np.abs(Y.cpu().numpy() - Yref).max() <= 1e-12 * scale
     del X
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("kind", ("f64", "f32", "lowp_f32", "lowp_bf16"))
+@pytest.mark.parametrize("r", (2, 4, 8))
+@pytest.mark.parametrize("k,kt", [(2, 0), (3, 0), (16, 0), (16, 4), (64, 0), (70, 0), (70, 32)])
+def test_spmm_column_records(kind, r, k, kt):
+    """The column-record schedule (SPC5_HIP_OPT_SPMM_COLUMNS: one X row
+    request per set column of a block, r running row sums) against the
+    dense product of the values the matrix holds, and against the
+    per-value schedule (agreement to rounding)."""
+    rp, col, val = clustered_csr(577, 430, seed=5 * r + k, empty_frac=0.1)
+    A, vt, vheld = _matrix_of_kind(kind, rp, col, val, r, 430)
+    rng = np.random.default_rng(k + 7 * r)
+    X = rng.uniform(-1, 1, (430, k)).astype(vt)
+    Y0 = rng.uniform(-1, 1, (577, k)).astype(vt)
+    A.spmm_columns = True
+    Yc = dev(Y0.copy())
+    A.spmm(dev(X), Yc, alpha=0.75, beta=1.25, kt=kt)
+    A.spmm_columns = False
+    Yv = dev(Y0.copy())
+    A.spmm(dev(X), Yv, alpha=0.75, beta=1.25, kt=kt)
+    Ad = dense_of(rp, col, vheld, 430)
+    ref = 0.75 * (Ad @ X.astype(np.float64)) + 1.25 * Y0
+    scale = max(np.abs(Ad).sum(axis=1).max() * np.abs(X).max(), np.abs(Y0).max())
+    tol = 1e-12 if vt == np.float64 else 1e-5
+    got_c = Yc.cpu().numpy().astype(np.float64)
+    assert np.abs(got_c - ref).max() / scale <= tol
+    assert np.abs(got_c - Yv.cpu().numpy().astype(np.float64)).max() / scale <= tol
+
+
+@pytest.mark.parametrize("kind", ("f64", "lowp_f32"))
+def test_spmm_column_records_stencil_tuned(kind):
+    """The multi-right-hand-side workload's matrix family (27-point stencil,
+    18^3) with r, kt and the record kind picked by autotune.tune_spmm over
+    r in {2, 4, 8}, k = 64, against scipy's product."""
+    import scipy.sparse as sp
+    from spc5_amd import autotune, synth
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import lowp
+    rp, col, val = synth.stencil27(18)
+    n = len(rp) - 1
+    d = (dev(rp), dev(col), dev(val))
+    mats = {}
+    for r in (2, 4, 8):
+        m = DeviceMatrix.from_csr(*d, r)
+        mats[r] = m if kind == "f64" else lowp(m, "f32")
+    (r, kt, cols), times = autotune.tune_spmm(mats, 64, reps=2)
+    assert (r, kt, cols) in times and len(times) >= 6
+    A = mats[r]
+    assert A.spmm_columns == cols
+    vt = np.float64 if kind == "f64" else np.float32
+    vheld = val if kind == "f64" else _rounded(val, "lowp_f32")
+    rng = np.random.default_rng(64)
+    X = rng.uniform(-1, 1, (n, 64)).astype(vt)
+    Y = dev(np.zeros((n, 64), vt))
+    A.spmm(dev(X), Y, alpha=1.0, beta=0.0, kt=kt)
+    ref = sp.csr_matrix((vheld, col, rp), shape=(n, n)) @ X.astype(np.float64)
+    err = np.abs(Y.cpu().numpy().astype(np.float64) - ref).max() / 54.001
+    assert err <= (1e-12 if kind == "f64" else 1e-5), (r, kt, cols, err)
