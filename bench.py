@@ -102,7 +102,7 @@ def copy_bandwidth():
 
 def stream_ceiling(read_bytes, write_bytes, tiles_kib=(8, 16, 32, 64), steps=20):
     """Rate of the library's read-dominated streaming probe (each wave reads
-    one contiguous tile with non-temporal 16-byte loads, as the product's
+    one contiguous tile with coalesced non-temporal loads, as the product's
     waves stream their tiles) with the byte mix of one product: the
     bandwidth a kernel that only streams these bytes reaches on this device.
     Returns (GB/s of the best tile size, {tile KiB: GB/s})."""
@@ -198,6 +198,7 @@ def solve_benchmark(n=160, rtol=1e-10):
     import torch
     from spc5_amd import synth
     from spc5_amd.device import DeviceMatrix
+    from spc5_amd import autotune
     from spc5_amd.solver import cg, ir_cg, lowp, mixed_cg
 
     rp, col, val = synth.stencil27(n)
@@ -208,10 +209,12 @@ def solve_benchmark(n=160, rtol=1e-10):
     bf = b.to(torch.float32)
 
     def fastest(cands, x, y):
-        """(r, row_sort) -> matrix; returns the key of the fastest product and all times."""
+        """(r, row_sort) -> matrix; returns the key of the fastest product and
+        all times.  Each candidate runs with its autotuned tile size."""
         times = {}
         for (rr, srt), m in cands.items():
             m.row_sort = srt
+            autotune.tune_tiles(m, reps=5)
             times[(rr, srt)] = time_spmv(m, x, y, 20, 3)[0]
         return min(times, key=times.get), times
 
@@ -226,7 +229,7 @@ def solve_benchmark(n=160, rtol=1e-10):
     r = k64[0]
     ms_64 = times64[k64]
     out = {"workload": f"stencil27 {n}^3 (N={N}, nnz={len(val)}), diag 26.001, b U[-1,1] seed 7", "rtol": rtol,
-           "fp64_matrix": {"r": r, "row_sort": k64[1], "spmv_ms": round(ms_64, 4),
+           "fp64_matrix": {"r": r, "row_sort": k64[1], "tile_values": A64.tile_target(), "spmv_ms": round(ms_64, 4),
                            "spmv_ms_by_shape": {keyname(k): round(v, 4) for k, v in times64.items()}}}
     log(f"[bench] fp64 SpMV {keyname(k64)} {ms_64:.4f} ms "
         f"(candidates {', '.join(f'{keyname(k)}: {v:.4f}' for k, v in sorted(times64.items()))})")
@@ -409,11 +412,13 @@ def batch_benchmark(count=2000, fill=0.3, steps=10, warmup=2):
         ys = [torch.zeros(m.nrows, dtype=m.vec_dtype, device="cuda") for m in ms]
         # row-sorted batches (used when every matrix shares r = 2 or 4) and
         # plain ones; the faster is the case's time
-        times, infos = {}, {}
+        times, infos, wave_times = {}, {}, {}
         for srt in (False, True):
             for m in ms:
                 m.row_sort = srt
-            b = Batch(ms, xs, ys)
+            # the persistent wave count is the autotuner's choice (timed candidates)
+            waves, wave_times[srt] = autotune.tune_batch_waves(lambda w: Batch(ms, xs, ys, waves=w), reps=3)
+            b = Batch(ms, xs, ys, waves=waves)
             times[srt] = _events_time(lambda: b.spmv(graph=True), steps, warmup)
             B = b.algorithmic_bytes()
             infos[srt] = b.info()
@@ -422,9 +427,14 @@ def batch_benchmark(count=2000, fill=0.3, steps=10, warmup=2):
         srt = min(times, key=times.get)
         t = times[srt]
         info = infos[srt]
+        tiles_hist = {}
+        for m in ms:
+            tiles_hist[m.tile_target()] = tiles_hist.get(m.tile_target(), 0) + 1
         case = {"dtype": tag, "ms": round(t * 1e3, 3), "gflops": round(2.0 * nnz / t / 1e9, 1),
                 "gbs": round(B / t / 1e9, 1), "frac_of_peak": round(B / t / 1e9 / HBM_PEAK_GBS, 4),
                 "bytes": B, "tiles": info["ntiles"], "persistent_waves": info["waves"], "launch": "hipGraph",
+                "waves_candidates_ms": {str(k): round(v * 1e3, 3) for k, v in wave_times[srt].items()},
+                "tile_values_histogram": {str(k): v for k, v in sorted(tiles_hist.items())},
                 "row_sort": srt, "ms_by_row_sort": {str(k): round(v * 1e3, 3) for k, v in times.items()}}
         out["cases"].append(case)
         log(f"[bench] batch {tag}: {t * 1e3:.2f} ms {case['gflops']} GFLOP/s {case['gbs']} GB/s "
@@ -452,6 +462,7 @@ def main():
     ap.add_argument("--cpu-max-rows", type=int, default=0)
     ap.add_argument("--no-copy", action="store_true")
     ap.add_argument("--no-ceiling", action="store_true", help="skip the streaming-probe ceiling")
+    ap.add_argument("--no-tune-tiles", action="store_true", help="default tile sizes (no tile autotuning)")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 variant of the SpMV matrix")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--solve-n", type=int, default=160)
@@ -485,6 +496,7 @@ def main():
     x = torch.from_numpy(synth.uniform(ncols, 3)).cuda()
     y = torch.zeros(n, dtype=torch.float64, device="cuda")
 
+    from spc5_amd import autotune
     formats = {}
     best = None
     for r in [int(s) for s in args.formats.split(",")]:
@@ -492,17 +504,23 @@ def main():
         m = DeviceMatrix.from_csr(d_rp, d_col, d_val, r)
         torch.cuda.synchronize()
         conv_s = time.perf_counter() - tc
+        # the tile size is the autotuner's per-matrix choice (timed candidates)
+        tile, tile_times = autotune.tune_tiles(m) if not args.no_tune_tiles else (0, {})
         ms, wall = time_spmv(m, x, y, args.steps, args.warmup)
         B = m.algorithmic_bytes()
         gbs = B / (ms * 1e-3) / 1e9
         gflops = 2.0 * nnz / (ms * 1e-3) / 1e9
         info = m.info()
         formats[m.format_name] = {"r": r, "nblocks": m.nblocks, "ntiles": info["ntiles"],
+                                  "tile_values": m.tile_target(), "tile_chosen_by": "autotune.tune_tiles",
+                                  "tile_candidates_ms": {str(k): round(v * 1e3, 5) for k, v in tile_times.items()},
                                   "bytes_per_spmv": B, "ms": round(ms, 5), "gflops": round(gflops, 2),
                                   "gbs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
                                   "convert_s": round(conv_s, 3), "timed_wall_s": round(wall, 3)}
         log(f"[bench] beta({r},8): {ms:.4f} ms  {gflops:.1f} GFLOP/s  {gbs:.0f} GB/s  "
-            f"({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)  nb={m.nblocks} tiles={info['ntiles']}")
+            f"({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)  nb={m.nblocks} tiles={info['ntiles']} "
+            f"(tile size {m.tile_target()} values; candidates "
+            f"{', '.join(f'{k}: {v * 1e3:.4f}' for k, v in tile_times.items())})")
         if best is None or ms < formats[best]["ms"]:
             best = m.format_name
         del m
@@ -538,8 +556,8 @@ def main():
         c_gbs, c_rates = stream_ceiling(bf["bytes_per_spmv"] - wbytes, wbytes)
         ceiling = {"gbs": c_gbs, "by_tile_kib": {str(k): v for k, v in c_rates.items()},
                    "read_bytes": bf["bytes_per_spmv"] - wbytes, "write_bytes": wbytes,
-                   "kernel": "k_stream_probe (one contiguous tile per wave, non-temporal 16-byte loads, 4 KiB "
-                             "in flight per wave)"}
+                   "kernel": "k_stream_probe (one contiguous tile per wave, coalesced non-temporal 8-byte "
+                             "loads, 3 KiB in flight per wave)"}
         log(f"[bench] streaming ceiling (read-dominated probe, product's byte mix): {c_gbs} GB/s {c_rates}")
     out = {
         "metric": METRIC,

@@ -152,6 +152,12 @@ int spc5_hip_mat_load(const char* path, spc5_hip_mat_t* out);
  * rows of a block share columns (stencils); results agree with the default
  * to rounding.  No counterpart in the reference. */
 #define SPC5_HIP_OPT_SPMM_COLUMNS 2
+/* TILE_VALUES (0, or 64 .. 2^24): values one wave's tile aims at; a tile
+ * also ends every 128 rows.  0 restores the default (nnz / 8192, clamped
+ * to [512, 8192]).  Rebuilds the tile plan (synchronises the device); a
+ * batch created afterwards uses the new tiles.  No counterpart in the
+ * reference (the work split of the GPU kernel). */
+#define SPC5_HIP_OPT_TILE_VALUES 3
 int spc5_hip_mat_set_option(spc5_hip_mat_t h, int option, int64_t value);
 
 /* y += A x on device pointers (x and y in the matrix's vector type).
@@ -318,8 +324,8 @@ int spc5_hip_synth_banded_fill(int64_t n, int64_t bw, uint64_t thr, uint64_t see
 
 /* Read-dominated streaming probe (no counterpart in the reference; a
  * measurement aid): every wave reads one contiguous tile of tile_bytes of
- * d_src with non-temporal 16-byte loads (4 KiB in flight per wave), as a
- * product's waves stream their tiles, then the waves write write_bytes of
+ * d_src with coalesced non-temporal 8-byte loads (3 KiB in flight per
+ * wave), as a product's waves stream their tiles, then the waves write write_bytes of
  * d_dst (which must hold at least 16 bytes even when write_bytes is 0).
  * Sizes are multiples of 16, tile_bytes of 1 KiB.  With a write share of
  * about 3 % it has the HBM mix of one SpMV, so its rate is the ceiling

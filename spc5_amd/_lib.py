@@ -18,6 +18,7 @@ SPC5_HIP_F32 = 1
 SPC5_HIP_BF16 = 2
 SPC5_HIP_OPT_ROW_SORT = 1
 SPC5_HIP_OPT_SPMM_COLUMNS = 2
+SPC5_HIP_OPT_TILE_VALUES = 3
 
 _P = C.c_void_p
 _I = C.c_int

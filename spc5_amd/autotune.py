@@ -1,29 +1,34 @@
-"""Autotuner of the batching form: r (rows per block) per matrix and the
-column chunk kt of multi-right-hand-side products.
+"""Autotuner of the SpMV and its batching form: per matrix, r (rows per
+block), the tile size of the product, the persistent wave count of a batch,
+and the column chunk and record kind of multi-right-hand-side products.
 
 r is chosen from a cost model evaluated on GPU block counts (one count pass
-per candidate, no conversion):
+per candidate, no conversion), one set of non-negative costs per r:
 
-    t(r) = a * bytes(r) + c * nblocks(r) + d * batches(r) + e_r * nnz
+    t(r) = a_r * nnz + b_r * nblocks(r) + c_r * batches(r)
 
-bytes(r) = nnz*sv + nblocks*(4 + r*MB) + (ceil(N/r)+1)*4 + ncols*sx + 2*N*sx
-is what one product must move; batches(r) = max(nblocks/64, nnz/cap) counts
-the kernel's batches (64 blocks or `cap` values each); e_r is the per-value
-cost of r's expansion and row sums.  The coefficients are fitted (least
-squares, one model for all r) by `calibrate()` from timed products of
+batches(r) = max(nblocks/64, nnz/cap) counts the kernel's batches (64 blocks
+or `cap` values each).  a_r prices the value stream and the per-value
+expansion and row sums of r's kernel, b_r the block headers and x
+requests, c_r the fixed work of a batch.  The costs are fitted by
+non-negative least squares (`fit_coefficients`) to timed products of
 synthetic matrices on the device and stored, with the raw timings, in
-tuning_mi355x.json.  Left out of the fit one matrix at a time, the model
-picked the measured best r for 18 of 24 matrices, 1.4 % slower than the
-best on average and 12.9 % at worst.  Without a calibration file it reduces
-to bytes at 5.5 TB/s plus a fixed per-block decode cost.  The
-reference's own CPU model (SPC5_find_best, cubic in values per block) is
-kept in spc5_amd.spc5 for API parity; its coefficients describe a CPU and are
-not used here.
+tuning_mi355x.json (`calibrate`).  Left out of the fit one matrix at a time
+(`leave_one_out`), the model picks the measured best r for 20 of the 24
+calibration matrices, 0.7 % slower than the best on average and 10.5 % at
+worst (a single cost set for all r, fitted by plain least squares with
+negative costs, managed 18).  Without a calibration file it reduces to
+bytes at 5.5 TB/s plus a fixed per-block decode cost.  The reference's own
+CPU model (SPC5_find_best, cubic in values per block) is kept in
+spc5_amd.spc5 for API parity; its coefficients describe a CPU and are not
+used here.
 
 `mode="measure"` converts the matrix with each candidate r and times it
-instead (exact, costs four conversions); `tune_spmm_kt` times the candidate
-column chunks of a product with k right-hand sides, and `tune_spmm` picks r,
-the chunk and the record kind (per value or per block column) by timing.
+instead (exact, costs four conversions).  `tune_tiles` times candidate tile
+sizes of one matrix's product, `tune_batch_waves` candidate persistent wave
+counts of a batch, `tune_spmm_kt` the column chunks of a product with k
+right-hand sides, and `tune_spmm` picks r, the chunk and the record kind
+(per value or per block column) by timing.
 """
 from __future__ import annotations
 
@@ -38,18 +43,6 @@ from .device import DeviceMatrix, block_count, default_vs, mask_bytes
 
 RS = (1, 2, 4, 8)
 DEFAULT_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_mi355x.json")
-# bytes at 5.5 TB/s plus 12 ps per block of decode work: used until calibrate() runs
-_DEFAULT = {"bytes": 1.0 / 5.5e12, "nblocks": 1.2e-11, "batches": 0.0, "nnz": {str(r): 0.0 for r in RS}}
-
-
-def load_coefficients(path: str = DEFAULT_FILE):
-    if os.path.exists(path):
-        with open(path) as f:
-            d = json.load(f)
-        c = d.get("coefficients")
-        if isinstance(c, dict) and "bytes" in c:
-            return c
-    return _DEFAULT
 
 
 def spmv_bytes(nrows, ncols, nnz, nb, r, value_bytes, vec_bytes, vs):
@@ -63,25 +56,73 @@ def batch_cap(value_bytes):
 
 
 def features(nrows, ncols, nnz, nb, r, value_bytes, vec_bytes, vs):
-    """[bytes, nblocks, batches, nnz if r == 1, .. 2, .. 4, .. 8]"""
+    """The model's terms of one (matrix, r): [nnz, nblocks, batches] in the
+    slots of r, zeros in the slots of the other r (3 per r, r = 1, 2, 4, 8)."""
     batches = max(nb / 64.0, nnz / float(batch_cap(value_bytes)))
-    return np.array([spmv_bytes(nrows, ncols, nnz, nb, r, value_bytes, vec_bytes, vs), nb, batches]
-                    + [float(nnz) if r == q else 0.0 for q in RS], dtype=np.float64)
+    out = np.zeros(3 * len(RS), dtype=np.float64)
+    i = RS.index(r)
+    out[3 * i: 3 * i + 3] = (float(nnz), float(nb), batches)
+    return out
+
+
+def _default_coefficients():
+    # bytes at 5.5 TB/s (values 8 B, headers ~5 B per block) plus 12 ps per block
+    return {str(r): {"value": 8.0 / 5.5e12, "block": (4 + r) / 5.5e12 + 1.2e-11, "batch": 0.0} for r in RS}
+
+
+def load_coefficients(path: str = DEFAULT_FILE):
+    if os.path.exists(path):
+        with open(path) as f:
+            d = json.load(f)
+        c = d.get("coefficients")
+        if isinstance(c, dict) and all(str(r) in c for r in RS):
+            return c
+    return _default_coefficients()
 
 
 def _model_vector(coeffs):
-    return np.array([coeffs["bytes"], coeffs["nblocks"], coeffs["batches"]] + [coeffs["nnz"][str(q)] for q in RS])
+    return np.array([coeffs[str(r)][k] for r in RS for k in ("value", "block", "batch")], dtype=np.float64)
+
+
+def _measurement_features(m):
+    return features(m["nrows"], m.get("ncols", m["nrows"]), m["nnz"], m["nblocks"], m["r"], 8, 8, 8)
 
 
 def fit_coefficients(measurements):
-    """Least-squares fit of the model to timed products: `measurements` is a
-    list of {r, nrows, ncols?, nnz, nblocks, seconds} (double values)."""
-    X = np.array([features(m["nrows"], m.get("ncols", m["nrows"]), m["nnz"], m["nblocks"], m["r"], 8, 8, 8)
-                  for m in measurements])
-    y = np.array([m["seconds"] for m in measurements])
-    sol, *_ = np.linalg.lstsq(X, y, rcond=None)
-    return {"bytes": float(sol[0]), "nblocks": float(sol[1]), "batches": float(sol[2]),
-            "nnz": {str(q): float(sol[3 + i]) for i, q in enumerate(RS)}}
+    """Non-negative least-squares fit of the per-r costs to timed products:
+    `measurements` is a list of {r, nrows, ncols?, nnz, nblocks, seconds}
+    (double values).  Columns are scaled to unit maximum for the solve."""
+    from scipy.optimize import nnls
+    X = np.array([_measurement_features(m) for m in measurements])
+    y = np.array([m["seconds"] for m in measurements], dtype=np.float64)
+    scale = np.abs(X).max(axis=0)
+    scale[scale == 0] = 1.0
+    w, _ = nnls(X / scale, y)
+    w = w / scale
+    return {str(r): {"value": float(w[3 * i]), "block": float(w[3 * i + 1]), "batch": float(w[3 * i + 2])}
+            for i, r in enumerate(RS)}
+
+
+def leave_one_out(measurements):
+    """Fit without each matrix in turn and compare the model's pick of r for
+    it with the measured best.  `measurements` holds every r of a matrix
+    (matrices are told apart by nrows, nnz).  Returns {agree, matrices,
+    mean_loss, worst_loss} (loss: the pick's time over the best, minus 1)."""
+    groups = {}
+    for m in measurements:
+        groups.setdefault((m["nrows"], m["nnz"]), []).append(m)
+    keys = list(groups)
+    agree, losses = 0, []
+    for k in keys:
+        coeffs = fit_coefficients([m for kk in keys if kk != k for m in groups[kk]])
+        w = _model_vector(coeffs)
+        pred = {m["r"]: float(np.dot(w, _measurement_features(m))) for m in groups[k]}
+        meas = {m["r"]: m["seconds"] for m in groups[k]}
+        rp, rm = min(pred, key=pred.get), min(meas, key=meas.get)
+        agree += rp == rm
+        losses.append(meas[rp] / meas[rm] - 1.0)
+    return {"agree": agree, "matrices": len(keys), "mean_loss": float(np.mean(losses)),
+            "worst_loss": float(np.max(losses))}
 
 
 def block_counts(rowptr, col, vs: int = 8):
@@ -149,8 +190,10 @@ def calibrate(matrices, path: str = DEFAULT_FILE, reps=20):
             del m
     coeffs = fit_coefficients(raw)
     with open(path, "w") as f:
-        json.dump({"device": torch.cuda.get_device_name(0), "time": time.time(), "coefficients": coeffs,
-                   "samples": len(matrices), "measurements": raw}, f, indent=1)
+        json.dump({"device": torch.cuda.get_device_name(0), "time": time.time(),
+                   "model": "t(r) = value_r*nnz + block_r*nblocks(r) + batch_r*batches(r), non-negative least squares",
+                   "coefficients": coeffs, "leave_one_out": leave_one_out(raw), "samples": len(matrices),
+                   "measurements": raw}, f, indent=1)
     return coeffs
 
 
@@ -193,6 +236,55 @@ def tune_spmm(mats, k: int, kt_candidates=None, reps=5):
                 if best_t is None or t < best_t:
                     best_t, best_cols = t, cols
         m.spmm_columns = best_cols
+    best = min(times, key=times.get)
+    return best, times
+
+
+def _time_fn(fn, reps=10, warm=2):
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps * 1e-3
+
+
+def tune_tiles(m: DeviceMatrix, candidates=(0, 1024, 2048, 4096, 8192), reps=10):
+    """Fastest tile size (values per wave's tile, SPC5_HIP_OPT_TILE_VALUES;
+    0 is the default rule) for the matrix's product; leaves the matrix set
+    to it.  Returns (best, {candidate: seconds})."""
+    x = torch.ones(m.ncols, dtype=m.vec_dtype, device="cuda")
+    y = torch.zeros(m.nrows, dtype=m.vec_dtype, device="cuda")
+    times = {}
+    for t in candidates:
+        m.tile_values = t
+        times[t] = _time_fn(lambda: m.spmv(x, y), reps)
+    best = min(times, key=times.get)
+    m.tile_values = best
+    return best, times
+
+
+def tune_batch_waves(make_batch, fractions=(0.5, 1.0, 1.5, 2.0), reps=5):
+    """Fastest persistent wave count of a batch: `make_batch(waves)` builds
+    a batch.Batch (waves = 0 is the kernel's resident count); candidates are
+    fractions of that count.  Returns (waves, {waves: seconds}); the
+    batches built here are closed."""
+    b0 = make_batch(0)
+    base = b0.info()["waves"]
+    b0.close()
+    times = {}
+    for f in fractions:
+        w = max(1, int(round(base * f)))
+        if w in times:
+            continue
+        b = make_batch(w)
+        times[b.info()["waves"]] = _time_fn(lambda: b.spmv(graph=True), reps)
+        b.close()
     best = min(times, key=times.get)
     return best, times
 

@@ -46,6 +46,9 @@ struct spc5_hip_batch {
     DeviceBuffer tile_mat;  // int32[ntiles]: matrix of each tile
     DeviceBuffer tile_loc;  // int32[ntiles]: tile index inside its matrix
     std::vector<spc5_hip_mat_t> mats;  // borrowed handles
+    // The tile lists the batch was built from: a matrix's plan may be
+    // rebuilt later (SPC5_HIP_OPT_TILE_VALUES); the batch keeps its own.
+    std::vector<std::shared_ptr<DeviceBuffer>> tiles_kept;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t capture_stream = nullptr;
@@ -231,6 +234,10 @@ int spc5_hip_batch_create(int nmats, const spc5_hip_mat_t* mats, const void* con
         e.y = d_y[i];
         e.tile_br = m->plan.ntiles ? m->plan.tile_br->as<const int>() : nullptr;
         e.tile_val = m->plan.ntiles ? m->plan.tile_val->as<const int>() : nullptr;
+        if (m->plan.ntiles) {
+            b->tiles_kept.push_back(m->plan.tile_br);
+            b->tiles_kept.push_back(m->plan.tile_val);
+        }
         e.nrows = m->nrows;
         e.ncols = m->ncols;
         e.r = m->r;

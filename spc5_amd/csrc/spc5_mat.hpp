@@ -99,6 +99,8 @@ int launch_spmv_f64vec(const spc5_hip_mat& m, const double* x, double* y, hipStr
                        const Epilogue& ep = Epilogue());
 // Build the tile plan of a handle (defined in spmv.hip).
 int build_plan(spc5_hip_mat& m, hipStream_t s);
+// Values per tile of the default plan (defined in spmv.hip).
+int default_tile_values(int64_t nnz);
 // Another tile list of the same matrix with at most `maxrows` rows per tile
 // (needs the value offsets of m.plan; defined in spmv.hip).
 int build_tiles(const spc5_hip_mat& m, Plan& p, int maxrows, int target, hipStream_t s);

@@ -261,10 +261,17 @@ int build_plan(spc5_hip_mat& m, hipStream_t s) {
         set_error("nnz (" + std::to_string(m.nnz) + ") differs from the mask popcount (" + std::to_string(hnnz) + ")");
         return SPC5_HIP_ERR_INVALID;
     }
-    int64_t target = m.nnz / 8192;
+    return build_tiles(m, p, kMaxRows, default_tile_values(m.nnz), s);
+}
+
+// Values per tile aimed at when no tile size is set: about 8192 tiles, at
+// least 512 and at most 8192 values each (a tile also ends every
+// kMaxRows rows).
+int default_tile_values(int64_t nnz) {
+    int64_t target = nnz / 8192;
     if (target < 512) target = 512;
     if (target > 8192) target = 8192;
-    return build_tiles(m, p, kMaxRows, (int)target, s);
+    return (int)target;
 }
 
 // Tile cut from the block-row value offsets: a tile starts where the running
@@ -382,6 +389,17 @@ int spc5_hip_mat_set_option(spc5_hip_mat_t h, int option, int64_t value) {
         SPC5_CHECK_ARG(value == 0 || value == 1, "SPMM_COLUMNS must be 0 or 1");
         h->spmm_cols = (int)value;
         return SPC5_HIP_OK;
+    case SPC5_HIP_OPT_TILE_VALUES: {
+        SPC5_CHECK_ARG(value == 0 || (value >= 64 && value <= (1 << 24)), "TILE_VALUES must be 0 or in [64, 2^24]");
+        const int target = value == 0 ? default_tile_values(h->nnz) : (int)value;
+        if (h->nbr == 0 || target == h->plan.target) return SPC5_HIP_OK;
+        Plan np;
+        np.valptr = h->plan.valptr;
+        const int rc = build_tiles(*h, np, kMaxRows, target, nullptr);
+        if (rc) return rc;
+        h->plan = np;  // shared buffers of the old plan stay alive while views or batches use them
+        return SPC5_HIP_OK;
+    }
     default:
         set_error("unknown option");
         return SPC5_HIP_ERR_INVALID;

@@ -55,39 +55,32 @@ __global__ void k_banded_fill(int64_t n, int64_t bw, unsigned long long thr, uns
 }
 
 
-// Read-dominated streaming probe: each wave reads one contiguous tile of
-// `tile` 16-byte words with non-temporal loads, 4 KiB per step (four
-// 16-byte loads per lane in flight), the way a product's waves stream their
-// tiles; then the waves write dst (m16 words) in a grid-stride loop.  With
-// a write share of a few percent this is the HBM mix of a product y = A x
-// (about 97 % reads), so its rate is the ceiling such a kernel can reach.
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
-__global__ __launch_bounds__(256) void k_stream_probe(const u32x4* __restrict__ src, int64_t n16, int64_t tile,
-                                                      u32x4* __restrict__ dst, int64_t m16) {
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int64_t t0 = wave * tile;
-    const int64_t t1 = t0 + tile < n16 ? t0 + tile : n16;
-    unsigned acc = 0;
-    int64_t i = t0 + lane;
-    for (; i + 192 < t1; i += 256) {
-        u32x4 v[4];
+// Read-dominated streaming probe: each wave (one per workgroup) reads one
+// contiguous tile of `tile` 8-byte words with non-temporal loads, 3 KiB per
+// step (six coalesced loads of one word per lane), the way a product's
+// waves stream their tiles; then the waves write dst (m8 words) in a
+// grid-stride loop.  With a write share of a few percent this is the HBM mix
+// of a product y = A x (about 97 % reads), so its rate is the ceiling such a
+// kernel can reach.
+__global__ __launch_bounds__(64) void k_stream_probe(const unsigned long long* __restrict__ src, int64_t n8,
+                                                     int64_t tile, unsigned long long* __restrict__ dst, int64_t m8) {
+    constexpr int kSteps = 6;
+    const int lane = threadIdx.x;
+    const int64_t t0 = (int64_t)blockIdx.x * tile;
+    const int64_t t1 = t0 + tile < n8 ? t0 + tile : n8;
+    unsigned long long acc = 0;
+    for (int64_t i = t0 + lane; i < t1; i += 64 * kSteps) {
+        unsigned long long v[kSteps];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(src + i + 64 * u);
+        for (int u = 0; u < kSteps; ++u) v[u] = i + 64 * u < t1 ? __builtin_nontemporal_load(src + i + 64 * u) : 0ull;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
-    }
-    for (; i < t1; i += 64) {
-        const u32x4 v = __builtin_nontemporal_load(src + i);
-        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+        for (int u = 0; u < kSteps; ++u) acc ^= v[u];
     }
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m16; j += stride)
-        dst[j] = u32x4{acc, (unsigned)j, 0u, 0u};
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m8; j += stride) dst[j] = acc ^ (unsigned)j;
     // The loads must not be dropped: the sum reaches memory on one
     // impossible-looking value (dst holds at least one word).
-    if (m16 == 0 && acc == 0x9e3779b9u) dst[0] = u32x4{acc, 1u, 0u, 0u};
+    if (m8 == 0 && acc == 0x9e3779b97f4a7c15ull) dst[0] = acc;
 }
 
 }  // namespace
@@ -130,13 +123,12 @@ int spc5_hip_stream_probe(const void* d_src, int64_t read_bytes, void* d_dst, in
     SPC5_CHECK_ARG(read_bytes >= 0 && write_bytes >= 0 && read_bytes % 16 == 0 && write_bytes % 16 == 0,
                    "sizes must be non-negative multiples of 16 bytes");
     SPC5_CHECK_ARG(tile_bytes >= 1024 && tile_bytes % 1024 == 0, "tile_bytes must be a positive multiple of 1 KiB");
-    const int64_t tile = tile_bytes / 16;
-    const int64_t waves = (read_bytes / 16 + tile - 1) / tile;
-    const int64_t groups = waves > 0 ? (waves + 3) / 4 : 1;
-    SPC5_CHECK_ARG(groups <= (1ll << 30), "too many tiles");
-    hipLaunchKernelGGL(k_stream_probe, dim3((unsigned)groups), dim3(256), 0, as_stream(stream),
-                       reinterpret_cast<const u32x4*>(d_src), read_bytes / 16, tile, reinterpret_cast<u32x4*>(d_dst),
-                       write_bytes / 16);
+    const int64_t tile = tile_bytes / 8;
+    const int64_t waves = (read_bytes / 8 + tile - 1) / tile;
+    SPC5_CHECK_ARG(waves <= (1ll << 30), "too many tiles");
+    hipLaunchKernelGGL(k_stream_probe, dim3((unsigned)(waves > 0 ? waves : 1)), dim3(64), 0, as_stream(stream),
+                       reinterpret_cast<const unsigned long long*>(d_src), read_bytes / 8, tile,
+                       reinterpret_cast<unsigned long long*>(d_dst), write_bytes / 8);
     SPC5_HIP_TRY(hipGetLastError());
     return SPC5_HIP_OK;
 }

@@ -13,7 +13,7 @@ import numpy as np
 import torch
 
 from ._lib import (SPC5_HIP_BF16, SPC5_HIP_F32, SPC5_HIP_F64, SPC5_HIP_OPT_ROW_SORT, SPC5_HIP_OPT_SPMM_COLUMNS,
-                   Spc5Error, check, lib)
+                   SPC5_HIP_OPT_TILE_VALUES, Spc5Error, check, lib)
 
 _VT = {torch.float64: SPC5_HIP_F64, torch.float32: SPC5_HIP_F32, torch.bfloat16: SPC5_HIP_BF16}
 _VEC_DTYPE = {SPC5_HIP_F64: torch.float64, SPC5_HIP_F32: torch.float32, SPC5_HIP_BF16: torch.float32}
@@ -195,6 +195,22 @@ class DeviceMatrix:
     def spmm_columns(self, on: bool):
         check(lib().spc5_hip_mat_set_option(self._h, SPC5_HIP_OPT_SPMM_COLUMNS, 1 if on else 0), "set_option")
         self._spmm_columns = bool(on)
+
+    @property
+    def tile_values(self) -> int:
+        """Values one wave's tile aims at (0: the default; see
+        SPC5_HIP_OPT_TILE_VALUES in include/spc5_hip.h)."""
+        return getattr(self, "_tile_values", 0)
+
+    @tile_values.setter
+    def tile_values(self, target: int):
+        check(lib().spc5_hip_mat_set_option(self._h, SPC5_HIP_OPT_TILE_VALUES, int(target)), "set_option")
+        self._tile_values = int(target)
+
+    def tile_target(self) -> int:
+        """Values per tile the current plan aims at (the set tile size, or
+        the default rule nnz / 8192 clamped to [512, 8192])."""
+        return self.tile_values or min(8192, max(512, self.nnz // 8192))
 
     @property
     def format_name(self):
