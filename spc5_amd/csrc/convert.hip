@@ -31,146 +31,285 @@ __global__ void k_check_rowptr(const int* __restrict__ rowptr, int64_t nrows, in
     if (bad) atomicOr(err, 1);
 }
 
-template <int R>
-struct Cursors {
-    int cur[R];
-    int end[R];
-    int head[R];  // column at the cursor, INT_MAX when the row is consumed
+// Groups: one wave converts kGroupRows = 64 consecutive rows, one lane per
+// row; the r lanes of a block-row form a lane group.  For every block the
+// group agrees on its start (the minimum of the r row heads, a shuffle
+// reduction), each lane consumes its own row's columns inside the block's
+// window, and a group prefix sum of the counts places the values (row-major
+// inside the block).  The group's column indices are staged in LDS with
+// coalesced loads when they fit (kStageValues); the fill pass builds the
+// group's packed headers and a destination map of its values in LDS and
+// writes both out with coalesced stores.  A group too large for the stage
+// reads its columns from global memory and writes directly (the same
+// arithmetic).
+constexpr int kGroupRows = 64;
+constexpr int kStageValues = 4096;   // staged column indices per group (16 KiB)
+constexpr int kStageHdrBytes = 8192; // staged header bytes per group
 
-    __device__ __forceinline__ void init(int64_t row0, int64_t nrows, const int* rowptr, const int* col) {
+template <int R>
+constexpr int group_blockrows() { return kGroupRows / R; }
+
+template <int R>
+__device__ __forceinline__ int group_min(int v) {
 #pragma unroll
-        for (int s = 0; s < R; ++s) {
-            if (row0 + s < nrows) {
-                cur[s] = rowptr[row0 + s];
-                end[s] = rowptr[row0 + s + 1];
-            } else {
-                cur[s] = end[s] = 0;
-            }
-            head[s] = cur[s] < end[s] ? col[cur[s]] : INT_MAX;
-        }
+    for (int off = 1; off < R; off <<= 1) {
+        const int o = __shfl_xor(v, off, R);
+        v = o < v ? o : v;
     }
-    // Smallest unconsumed column; false when every row is consumed.
-    __device__ __forceinline__ bool next_start(int& start) const {
-        bool any = false;
-        start = INT_MAX;
+    return v;
+}
+template <int R>
+__device__ __forceinline__ int group_sum(int v) {
 #pragma unroll
-        for (int s = 0; s < R; ++s) {
-            if (cur[s] < end[s]) {
-                any = true;
-                start = head[s] < start ? head[s] : start;
-            }
-        }
-        return any;
+    for (int off = 1; off < R; off <<= 1) v += __shfl_xor(v, off, R);
+    return v;
+}
+// Exclusive prefix sum over the lanes of a group (lane t: sum over t' < t).
+template <int R>
+__device__ __forceinline__ int group_excl_scan(int v, int t) {
+    int incl = v;
+#pragma unroll
+    for (int off = 1; off < R; off <<= 1) {
+        const int o = __shfl_up(incl, off, R);
+        if (t >= off) incl += o;
+    }
+    return incl - v;
+}
+
+// One row's cursor.  `col` is indexed by global value position minus
+// `base` (0 for the global array, the group's first value for the stage).
+struct RowCursor {
+    int cur, end, head;  // head: column at the cursor, INT_MAX once consumed
+    template <class C>
+    __device__ __forceinline__ void init(int b, int e, C col, int base) {
+        cur = b;
+        end = e;
+        head = b < e ? col[b - base] : INT_MAX;
     }
 };
 
-// Blocks per block-row.  Also checks the precondition of the format
-// (src/spc5.hpp:165-228 assumes it without a check): columns are
-// non-negative and strictly increasing inside every row.  A violation sets
-// bit 0 of *err; the fill pass is then never run.
-template <int R, int VS>
-__global__ void k_convert_count(int64_t nrows, int64_t nbr, const int* __restrict__ rowptr,
-                                const int* __restrict__ col, int* __restrict__ cnt_out, int* __restrict__ err) {
-    const int64_t br = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (br >= nbr) return;
-    Cursors<R> c;
-    c.init(br * R, nrows, rowptr, col);
-    bool bad = false;
-#pragma unroll
-    for (int s = 0; s < R; ++s) bad |= c.head[s] < 0;
-    int nb = 0;
-    int start;
-    while (c.next_start(start)) {
-        ++nb;
-        const int64_t lim = (int64_t)start + VS;
-#pragma unroll
-        for (int s = 0; s < R; ++s) {
-            while (c.cur[s] < c.end[s] && c.head[s] < lim) {
-                const int prev = c.head[s];
-                ++c.cur[s];
-                c.head[s] = c.cur[s] < c.end[s] ? col[c.cur[s]] : INT_MAX;
-                bad |= c.cur[s] < c.end[s] && c.head[s] <= prev;
+// Walks the block-rows of a group; for each block of a lane's block-row
+// calls on_block(k, start, t, first, n, mask) on every lane of the group: k
+// is the block's index in its block-row, t the lane's row in the block, first
+// the value position (block order) of the lane's first value in the block,
+// n its count and mask its VS-bit row mask; the lane's values were at
+// source positions [src0, src0 + n).  Returns whether a lane saw columns
+// that break the format's precondition (non-negative, strictly increasing).
+template <int R, int VS, class C, class F>
+__device__ __forceinline__ bool walk_group(int rb, int re, C col, int base, F on_block) {
+    const int lane = threadIdx.x;
+    const int t = lane % R;
+    RowCursor c;
+    c.init(rb, re, col, base);
+    bool bad = c.head < 0;
+    // the block-row's first value: the row pointer of its row 0
+    int vbase = __shfl(rb, lane - t, 64);
+    int k = 0;
+    for (;;) {
+        const int start = group_min<R>(c.head);
+        if (!ballot64(start != INT_MAX)) break;  // every block-row of the wave is done
+        if (start != INT_MAX) {
+            const int64_t lim = (int64_t)start + VS;
+            const int src0 = c.cur;
+            unsigned mask = 0;
+            int n = 0;
+            while (c.cur < c.end && c.head < lim) {
+                mask |= 1u << (unsigned)(c.head - start);
+                const int prev = c.head;
+                ++c.cur;
+                ++n;
+                c.head = c.cur < c.end ? col[c.cur - base] : INT_MAX;
+                bad |= c.cur < c.end && c.head <= prev;
             }
+            const int before = group_excl_scan<R>(n, t);
+            on_block(k, start, t, vbase + before, n, mask, src0);
+            vbase += group_sum<R>(n);
+            ++k;
         }
     }
-    cnt_out[br] = nb;
+    return bad;
+}
+
+// Coalesced copy of n words to LDS, 8 loads in flight per lane per step (a
+// load-then-store loop would wait for every load in turn).
+template <class T>
+__device__ __forceinline__ void stage_words(const T* __restrict__ src, T* dst, int n, int lane) {
+    constexpr int U = 8;
+    for (int i0 = 0; i0 < n; i0 += 64 * U) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + lane + 64 * u;
+            v[u] = i < n ? src[i] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + lane + 64 * u;
+            if (i < n) dst[i] = v[u];
+        }
+    }
+}
+
+// Group row pointers into LDS: srp[i] = rowptr[row0 + i] for the group's
+// rows and one past them (clamped to the matrix).
+__device__ __forceinline__ int64_t stage_rowptr(const int* __restrict__ rowptr, int64_t row0, int64_t nrows, int* srp,
+                                                int lane) {
+    const int64_t rowE = row0 + kGroupRows < nrows ? row0 + kGroupRows : nrows;
+    for (int64_t i = lane; i <= rowE - row0; i += 64) srp[i] = rowptr[row0 + i];
+    wave_sync();
+    return rowE;
+}
+
+// Blocks per block-row (the count pass).  A violation of the column
+// precondition sets bit 0 of *err; the fill pass is then never run.
+template <int R, int VS>
+__global__ __launch_bounds__(64) void k_convert_count(int64_t nrows, int64_t nbr, const int* __restrict__ rowptr,
+                                                      const int* __restrict__ col, int* __restrict__ cnt_out,
+                                                      int* __restrict__ err) {
+    constexpr int BR = group_blockrows<R>();
+    __shared__ int srp[kGroupRows + 1];
+    __shared__ int scol[kStageValues];
+    const int lane = threadIdx.x;
+    const int64_t br0 = (int64_t)blockIdx.x * BR;
+    const int64_t row0 = br0 * R;
+    const int64_t rowE = stage_rowptr(rowptr, row0, nrows, srp, lane);
+    const int v0 = srp[0];
+    const int nv = srp[rowE - row0] - v0;
+    const bool staged = nv <= kStageValues;
+    if (staged) {
+        stage_words(col + v0, scol, nv, lane);
+        wave_sync();
+    }
+    const int64_t row = row0 + lane;
+    const int rb = srp[(row < nrows ? row : nrows) - row0];
+    const int re = srp[(row + 1 < nrows ? row + 1 : nrows) - row0];
+    int nb = 0;
+    auto on_block = [&](int, int, int, int, int, unsigned, int) { ++nb; };
+    const bool bad = staged ? walk_group<R, VS>(rb, re, scol, v0, on_block) : walk_group<R, VS>(rb, re, col, 0, on_block);
+    const int64_t br = br0 + lane / R;
+    if (lane % R == 0 && br < nbr) cnt_out[br] = nb;
     if (bad) atomicOr(err, 1);
 }
 
 template <class W, int R, int VS, int MB>
-__global__ void k_convert_fill(int64_t nrows, int64_t nbr, const int* __restrict__ rowptr,
-                               const int* __restrict__ col, const W* __restrict__ vin,
-                               const int* __restrict__ bptr, int64_t nblocks, unsigned char* __restrict__ hdr,
-                               W* __restrict__ vout) {
+__global__ __launch_bounds__(64) void k_convert_fill(int64_t nrows, int64_t nbr, const int* __restrict__ rowptr,
+                                                     const int* __restrict__ col, const W* __restrict__ vin,
+                                                     const int* __restrict__ bptr, int64_t nblocks,
+                                                     unsigned char* __restrict__ hdr, W* __restrict__ vout) {
+    constexpr int BR = group_blockrows<R>();
     constexpr int H = 4 + R * MB;
-    constexpr int NW = (R * MB + 7) / 8;  // 64-bit words of the combined mask
-    const int64_t br = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (br >= nbr) return;
-    const int64_t row0 = br * R;
-    Cursors<R> c;
-    c.init(row0, nrows, rowptr, col);
-    int64_t bo = bptr[br];
-    const int64_t bend = bptr[br + 1] < nblocks ? bptr[br + 1] : nblocks;
-    int64_t vo = rowptr[row0];
-    int start;
-    while (c.next_start(start)) {
-        if (bo >= bend) break;  // inconsistent pointers: never write past the block-row
-        const int64_t lim = (int64_t)start + VS;
-        unsigned long long mw[NW];
-#pragma unroll
-        for (int k = 0; k < NW; ++k) mw[k] = 0ull;
-#pragma unroll
-        for (int s = 0; s < R; ++s) {
-            while (c.cur[s] < c.end[s] && c.head[s] < lim) {
-                // In range for sorted columns (checked by the count pass); the
-                // guard only keeps a direct call on unsorted input in bounds.
-                const unsigned off = (unsigned)(c.head[s] - start);
-                const int bit = s * VS + (int)(off < (unsigned)VS ? off : 0u);
-                mw[bit >> 6] |= 1ull << (bit & 63);
-                vout[vo++] = vin[c.cur[s]];
-                ++c.cur[s];
-                c.head[s] = c.cur[s] < c.end[s] ? col[c.cur[s]] : INT_MAX;
+    __shared__ int srp[kGroupRows + 1];
+    __shared__ int scol[kStageValues];
+    __shared__ unsigned short dmap[kStageValues];
+    __shared__ __align__(16) unsigned char shdr[kStageHdrBytes + 4];
+    const int lane = threadIdx.x;
+    const int t = lane % R;
+    const int64_t br0 = (int64_t)blockIdx.x * BR;
+    const int64_t row0 = br0 * R;
+    const int64_t brE = br0 + BR < nbr ? br0 + BR : nbr;
+    const int64_t rowE = stage_rowptr(rowptr, row0, nrows, srp, lane);
+    const int v0 = srp[0];
+    const int nv = srp[rowE - row0] - v0;
+    const int64_t blk0 = bptr[br0];
+    const int64_t blkE = bptr[brE] < nblocks ? bptr[brE] : nblocks;
+    const int64_t g0 = blk0 * H, g1 = blkE * H;  // the group's header bytes
+    const int align = (int)(g0 & 3);
+    const bool staged = nv <= kStageValues && (g1 - g0) + align <= kStageHdrBytes;
+    const int64_t row = row0 + lane;
+    const int rb = srp[(row < nrows ? row : nrows) - row0];
+    const int re = srp[(row + 1 < nrows ? row + 1 : nrows) - row0];
+    const int64_t br = br0 + lane / R;
+    int nblk = 0, bo = 0;  // blocks of the lane's block-row, its first block's index in the group
+    if (br < nbr) {
+        const int64_t b_beg = bptr[br];
+        const int64_t b_end = bptr[br + 1] < nblocks ? bptr[br + 1] : nblocks;
+        nblk = (int)(b_end > b_beg ? b_end - b_beg : 0);
+        bo = (int)(b_beg - blk0);
+    }
+    if (!staged) {
+        // too large for the stage: global columns, direct writes
+        walk_group<R, VS>(rb, re, col, 0, [&](int k, int start, int tt, int first, int n, unsigned mask, int src0) {
+            if (k >= nblk) return;  // inconsistent pointers: never write past the block-row
+            unsigned char* h = hdr + (blk0 + bo + k) * H;
+            if (tt == 0) __builtin_memcpy(h, &start, 4);
+            __builtin_memcpy(h + 4 + tt * MB, &mask, MB);
+            for (int j = 0; j < n; ++j) vout[first + j] = vin[src0 + j];
+        });
+        return;
+    }
+    stage_words(col + v0, scol, nv, lane);
+    wave_sync();
+    walk_group<R, VS>(rb, re, scol, v0, [&](int k, int start, int tt, int first, int n, unsigned mask, int src0) {
+        if (k >= nblk) return;
+        unsigned char* h = shdr + align + (bo + k) * H;
+        if (tt == 0) __builtin_memcpy(h, &start, 4);
+        __builtin_memcpy(h + 4 + tt * MB, &mask, MB);
+        for (int j = 0; j < n; ++j) dmap[src0 - v0 + j] = (unsigned short)(first - v0 + j);
+    });
+    wave_sync();
+    (void)t;
+    // headers: whole dwords of the destination as dword stores, the partial
+    // first and last dwords byte by byte (their other bytes are another
+    // group's)
+    {
+        const int64_t w0 = g0 >> 2, w1 = (g1 + 3) >> 2;  // destination dwords touched
+        unsigned* hw = reinterpret_cast<unsigned*>(hdr);
+        const unsigned* sw = reinterpret_cast<const unsigned*>(shdr);
+        for (int64_t w = w0 + lane; w < w1; w += 64) {
+            const int li = (int)(w - w0);
+            const int64_t bb = w * 4;  // first destination byte of this dword
+            if (bb >= g0 && bb + 4 <= g1) {
+                hw[w] = sw[li];
+            } else {
+                for (int q = 0; q < 4; ++q)
+                    if (bb + q >= g0 && bb + q < g1) hdr[bb + q] = shdr[li * 4 + q];
             }
         }
-        unsigned char* h = hdr + bo * H;
-        __builtin_memcpy(h, &start, 4);
-        __builtin_memcpy(h + 4, mw, R * MB);
-        ++bo;
+    }
+    // values: read in CSR order (8 loads in flight per lane), written to
+    // their block-order positions
+    {
+        constexpr int U = 8;
+        for (int i0 = 0; i0 < nv; i0 += 64 * U) {
+            W v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + lane + 64 * u;
+                v[u] = i < nv ? vin[v0 + i] : W(0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + lane + 64 * u;
+                if (i < nv) vout[v0 + dmap[i]] = v[u];
+            }
+        }
     }
 }
 
 template <int R, int VS>
 void launch_count(int64_t nrows, int64_t nbr, const int* rowptr, const int* col, int* out, int* err,
                   hipStream_t s) {
-    const unsigned g = (unsigned)((nbr + 255) / 256);
-    hipLaunchKernelGGL((k_convert_count<R, VS>), dim3(g), dim3(256), 0, s, nrows, nbr, rowptr, col, out, err);
+    const unsigned g = (unsigned)((nbr + group_blockrows<R>() - 1) / group_blockrows<R>());
+    hipLaunchKernelGGL((k_convert_count<R, VS>), dim3(g), dim3(64), 0, s, nrows, nbr, rowptr, col, out, err);
 }
 
 template <class W, int VS, int MB>
 int launch_fill_r(int r, int64_t nrows, int64_t nbr, const int* rowptr, const int* col, const void* vin,
                   const int* bptr, int64_t nblocks, unsigned char* hdr, void* vout, hipStream_t s) {
-    const unsigned g = (unsigned)((nbr + 255) / 256);
     const W* vi = reinterpret_cast<const W*>(vin);
     W* vo = reinterpret_cast<W*>(vout);
+#define SPC5_FILL_LAUNCH(R)                                                                                    \
+    hipLaunchKernelGGL((k_convert_fill<W, R, VS, MB>), dim3((unsigned)((nbr + group_blockrows<R>() - 1) /        \
+                                                                       group_blockrows<R>())),                  \
+                       dim3(64), 0, s, nrows, nbr, rowptr, col, vi, bptr, nblocks, hdr, vo)
     switch (r) {
-    case 1:
-        hipLaunchKernelGGL((k_convert_fill<W, 1, VS, MB>), dim3(g), dim3(256), 0, s, nrows, nbr, rowptr, col, vi, bptr,
-                           nblocks, hdr, vo);
-        break;
-    case 2:
-        hipLaunchKernelGGL((k_convert_fill<W, 2, VS, MB>), dim3(g), dim3(256), 0, s, nrows, nbr, rowptr, col, vi, bptr,
-                           nblocks, hdr, vo);
-        break;
-    case 4:
-        hipLaunchKernelGGL((k_convert_fill<W, 4, VS, MB>), dim3(g), dim3(256), 0, s, nrows, nbr, rowptr, col, vi, bptr,
-                           nblocks, hdr, vo);
-        break;
-    case 8:
-        hipLaunchKernelGGL((k_convert_fill<W, 8, VS, MB>), dim3(g), dim3(256), 0, s, nrows, nbr, rowptr, col, vi, bptr,
-                           nblocks, hdr, vo);
-        break;
+    case 1: SPC5_FILL_LAUNCH(1); break;
+    case 2: SPC5_FILL_LAUNCH(2); break;
+    case 4: SPC5_FILL_LAUNCH(4); break;
+    case 8: SPC5_FILL_LAUNCH(8); break;
     default: set_error("r must be 1, 2, 4 or 8"); return SPC5_HIP_ERR_INVALID;
     }
+#undef SPC5_FILL_LAUNCH
     SPC5_HIP_TRY(hipGetLastError());
     return SPC5_HIP_OK;
 }
