@@ -37,41 +37,85 @@ __global__ void k_check_rowptr(const int* __restrict__ rowptr, int64_t nrows, in
 // reduction), each lane consumes its own row's columns inside the block's
 // window, and a group prefix sum of the counts places the values (row-major
 // inside the block).  The group's column indices are staged in LDS with
-// coalesced loads when they fit (kStageValues); the fill pass builds the
-// group's packed headers and a destination map of its values in LDS and
-// writes both out with coalesced stores.  A group too large for the stage
-// reads its columns from global memory and writes directly (the same
-// arithmetic).
+// coalesced loads when they fit the stage; the fill pass builds the group's
+// packed headers and a destination map of its values in LDS and writes both
+// out with coalesced stores.  A group too large for the stage reads its
+// columns from global memory and writes directly (the same arithmetic).
+//
+// The stage is dynamic LDS sized at launch from the matrix's mean values and
+// header bytes per group (half as much again as the mean, at least 256 more):
+// a small stage keeps many waves resident per CU, which hides the latency of
+// each wave's few dependent memory round trips.
 constexpr int kGroupRows = 64;
-constexpr int kStageValues = 4096;   // staged column indices per group (16 KiB)
-constexpr int kStageHdrBytes = 8192; // staged header bytes per group
+constexpr int kPrefetch = 32;             // words per lane loaded at once: 2048 per group
+constexpr int kMaxStageValues = 8192;     // 32 KiB of staged columns
+constexpr int kMaxStageHdrBytes = 16384;  // staged header bytes
 
 template <int R>
 constexpr int group_blockrows() { return kGroupRows / R; }
 
+int64_t round_up64(int64_t v) { return (v + 63) / 64 * 64; }
+
+// Staged values per group: the mean plus half of it (at least 256 more).
+int stage_values(int64_t nnz, int64_t ngroups) {
+    const int64_t avg = ngroups > 0 ? (nnz + ngroups - 1) / ngroups : 0;
+    int64_t s = round_up64(avg + (avg / 2 > 256 ? avg / 2 : 256));
+    if (s < 512) s = 512;
+    return (int)(s < kMaxStageValues ? s : kMaxStageValues);
+}
+
+// Staged header bytes per group, by the same rule.
+int stage_header_bytes(int64_t nblocks, int h, int64_t ngroups) {
+    const int64_t avg = ngroups > 0 ? (nblocks * h + ngroups - 1) / ngroups : 0;
+    int64_t s = round_up64(avg + (avg / 2 > 256 ? avg / 2 : 256));
+    if (s < 512) s = 512;
+    return (int)(s < kMaxStageHdrBytes ? s : kMaxStageHdrBytes);
+}
+
+// Lane-group collectives over R = 1, 2, 4 or 8 consecutive lanes, as DPP
+// moves inside 16-lane rows (no LDS round trip): quad_perm [1,0,3,2] and
+// [2,3,0,1] exchange within pairs and quads, row_half_mirror (lane i of an
+// 8-lane half-row reads lane 7 - i) joins the two quads of an 8-lane group
+// once each quad holds its own result, and row_shr:n shifts for the scan.
+// Every lane of a group is active whenever one of them calls these.
+constexpr int kDppQuadXor1 = 0xB1;
+constexpr int kDppQuadXor2 = 0x4E;
+constexpr int kDppRowHalfMirror = 0x141;
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+
 template <int R>
 __device__ __forceinline__ int group_min(int v) {
-#pragma unroll
-    for (int off = 1; off < R; off <<= 1) {
-        const int o = __shfl_xor(v, off, R);
-        v = o < v ? o : v;
-    }
+    if constexpr (R >= 2) v = min(v, dpp_i32<kDppQuadXor1>(v));
+    if constexpr (R >= 4) v = min(v, dpp_i32<kDppQuadXor2>(v));
+    if constexpr (R >= 8) v = min(v, dpp_i32<kDppRowHalfMirror>(v));
     return v;
 }
 template <int R>
 __device__ __forceinline__ int group_sum(int v) {
-#pragma unroll
-    for (int off = 1; off < R; off <<= 1) v += __shfl_xor(v, off, R);
+    if constexpr (R >= 2) v += dpp_i32<kDppQuadXor1>(v);
+    if constexpr (R >= 4) v += dpp_i32<kDppQuadXor2>(v);
+    if constexpr (R >= 8) v += dpp_i32<kDppRowHalfMirror>(v);
     return v;
 }
 // Exclusive prefix sum over the lanes of a group (lane t: sum over t' < t).
 template <int R>
 __device__ __forceinline__ int group_excl_scan(int v, int t) {
     int incl = v;
-#pragma unroll
-    for (int off = 1; off < R; off <<= 1) {
-        const int o = __shfl_up(incl, off, R);
-        if (t >= off) incl += o;
+    if constexpr (R >= 2) {
+        const int o = dpp_i32<kDppRowShr1>(incl);
+        if (t >= 1) incl += o;
+    }
+    if constexpr (R >= 4) {
+        const int o = dpp_i32<kDppRowShr2>(incl);
+        if (t >= 2) incl += o;
+    }
+    if constexpr (R >= 8) {
+        const int o = dpp_i32<kDppRowShr4>(incl);
+        if (t >= 4) incl += o;
     }
     return incl - v;
 }
@@ -89,12 +133,13 @@ struct RowCursor {
 };
 
 // Walks the block-rows of a group; for each block of a lane's block-row
-// calls on_block(k, start, t, first, n, mask) on every lane of the group: k
-// is the block's index in its block-row, t the lane's row in the block, first
-// the value position (block order) of the lane's first value in the block,
-// n its count and mask its VS-bit row mask; the lane's values were at
-// source positions [src0, src0 + n).  Returns whether a lane saw columns
-// that break the format's precondition (non-negative, strictly increasing).
+// calls on_block(k, start, t, first, n, mask, src0) on every lane of the
+// group: k is the block's index in its block-row, t the lane's row in the
+// block, first the value position (block order) of the lane's first value in
+// the block, n its count and mask its VS-bit row mask; the lane's values were
+// at source positions [src0, src0 + n), whose columns the walk never reads
+// again.  Returns whether a lane saw columns that break the format's
+// precondition (non-negative, strictly increasing).
 template <int R, int VS, class C, class F>
 __device__ __forceinline__ bool walk_group(int rb, int re, C col, int base, F on_block) {
     const int lane = threadIdx.x;
@@ -150,85 +195,143 @@ __device__ __forceinline__ void stage_words(const T* __restrict__ src, T* dst, i
     }
 }
 
-// Group row pointers into LDS: srp[i] = rowptr[row0 + i] for the group's
-// rows and one past them (clamped to the matrix).
-__device__ __forceinline__ int64_t stage_rowptr(const int* __restrict__ rowptr, int64_t row0, int64_t nrows, int* srp,
-                                                int lane) {
-    const int64_t rowE = row0 + kGroupRows < nrows ? row0 + kGroupRows : nrows;
-    for (int64_t i = lane; i <= rowE - row0; i += 64) srp[i] = rowptr[row0 + i];
+// Raw buffer loads of one word type (a read past the resource's range
+// returns 0 and touches no memory).
+template <class T>
+struct RawWord;
+template <>
+struct RawWord<int> {
+    __device__ __forceinline__ static int get(__amdgpu_buffer_rsrc_t r, int off) {
+        return (int)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+    }
+};
+template <>
+struct RawWord<unsigned int> {
+    __device__ __forceinline__ static unsigned int get(__amdgpu_buffer_rsrc_t r, int off) {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+    }
+};
+template <>
+struct RawWord<unsigned long long> {
+    __device__ __forceinline__ static unsigned long long get(__amdgpu_buffer_rsrc_t r, int off) {
+        return __builtin_bit_cast(unsigned long long, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    }
+};
+template <>
+struct RawWord<unsigned short> {
+    __device__ __forceinline__ static unsigned short get(__amdgpu_buffer_rsrc_t r, int off) {
+        return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+    }
+};
+
+// v[u] = src[lane + 64 u] for u < kPrefetch (0 past n), through a buffer
+// resource of n words: every load unconditional and one lane offset for all,
+// so all are in flight at once at little register cost.
+template <class T>
+__device__ __forceinline__ void prefetch_words(const T* __restrict__ src, int n, int lane, T (&v)[kPrefetch]) {
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(src), (short)0, n * (int)sizeof(T), kRawBufferFlags);
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) v[u] = RawWord<T>::get(r, (lane + 64 * u) * (int)sizeof(T));
+}
+
+// Stages n column indices in LDS: the first 64 kPrefetch from registers
+// already loaded, the rest with stage_words.
+__device__ __forceinline__ void stage_columns(const int* __restrict__ src, int* dst, int n, int lane,
+                                              const int (&v)[kPrefetch]) {
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+        const int i = lane + 64 * u;
+        if (i < n) dst[i] = v[u];
+    }
+    if (n > 64 * kPrefetch) stage_words(src + 64 * kPrefetch, dst + 64 * kPrefetch, n - 64 * kPrefetch, lane);
     wave_sync();
-    return rowE;
+}
+
+// A group's rows, one per lane: [rb, re) are the lane's values (rows past the
+// matrix are empty at its end), v0 the group's first value, nv its count.
+struct GroupRows {
+    int rb, re, v0, nv;
+};
+__device__ __forceinline__ GroupRows group_rows(const int* __restrict__ rowptr, int64_t row0, int64_t nrows,
+                                                int lane) {
+    const int64_t row = row0 + lane;
+    GroupRows g;
+    g.rb = rowptr[row < nrows ? row : nrows];
+    g.re = rowptr[row + 1 < nrows ? row + 1 : nrows];
+    g.v0 = __shfl(g.rb, 0, 64);
+    g.nv = __shfl(g.re, 63, 64) - g.v0;
+    return g;
 }
 
 // Blocks per block-row (the count pass).  A violation of the column
-// precondition sets bit 0 of *err; the fill pass is then never run.
+// precondition sets bit 0 of *err; the fill pass is then never run.  Dynamic
+// LDS: sv staged column indices.
 template <int R, int VS>
 __global__ __launch_bounds__(64) void k_convert_count(int64_t nrows, int64_t nbr, const int* __restrict__ rowptr,
                                                       const int* __restrict__ col, int* __restrict__ cnt_out,
-                                                      int* __restrict__ err) {
+                                                      int* __restrict__ err, int sv) {
     constexpr int BR = group_blockrows<R>();
-    __shared__ int srp[kGroupRows + 1];
-    __shared__ int scol[kStageValues];
+    extern __shared__ __align__(16) unsigned char smem[];
+    int* scol = reinterpret_cast<int*>(smem);
     const int lane = threadIdx.x;
     const int64_t br0 = (int64_t)blockIdx.x * BR;
-    const int64_t row0 = br0 * R;
-    const int64_t rowE = stage_rowptr(rowptr, row0, nrows, srp, lane);
-    const int v0 = srp[0];
-    const int nv = srp[rowE - row0] - v0;
-    const bool staged = nv <= kStageValues;
-    if (staged) {
-        stage_words(col + v0, scol, nv, lane);
-        wave_sync();
-    }
-    const int64_t row = row0 + lane;
-    const int rb = srp[(row < nrows ? row : nrows) - row0];
-    const int re = srp[(row + 1 < nrows ? row + 1 : nrows) - row0];
+    const GroupRows g = group_rows(rowptr, br0 * R, nrows, lane);
     int nb = 0;
     auto on_block = [&](int, int, int, int, int, unsigned, int) { ++nb; };
-    const bool bad = staged ? walk_group<R, VS>(rb, re, scol, v0, on_block) : walk_group<R, VS>(rb, re, col, 0, on_block);
+    bool bad;
+    if (g.nv <= sv) {
+        if (g.nv > 0) {
+            int c[kPrefetch];
+            prefetch_words(col + g.v0, g.nv, lane, c);
+            stage_columns(col + g.v0, scol, g.nv, lane, c);
+        }
+        bad = walk_group<R, VS>(g.rb, g.re, scol, g.v0, on_block);
+    } else {
+        bad = walk_group<R, VS>(g.rb, g.re, col, 0, on_block);
+    }
     const int64_t br = br0 + lane / R;
     if (lane % R == 0 && br < nbr) cnt_out[br] = nb;
     if (bad) atomicOr(err, 1);
 }
 
+// Headers and values of the groups' blocks.  Dynamic LDS: sv staged column
+// indices, overwritten in place by the destination of each value once the
+// walk has consumed its column, then shb + 4 header bytes (the group's
+// headers, offset by the byte alignment of their destination).
 template <class W, int R, int VS, int MB>
 __global__ __launch_bounds__(64) void k_convert_fill(int64_t nrows, int64_t nbr, const int* __restrict__ rowptr,
                                                      const int* __restrict__ col, const W* __restrict__ vin,
                                                      const int* __restrict__ bptr, int64_t nblocks,
-                                                     unsigned char* __restrict__ hdr, W* __restrict__ vout) {
+                                                     unsigned char* __restrict__ hdr, W* __restrict__ vout, int sv,
+                                                     int shb) {
     constexpr int BR = group_blockrows<R>();
     constexpr int H = 4 + R * MB;
-    __shared__ int srp[kGroupRows + 1];
-    __shared__ int scol[kStageValues];
-    __shared__ unsigned short dmap[kStageValues];
-    __shared__ __align__(16) unsigned char shdr[kStageHdrBytes + 4];
+    extern __shared__ __align__(16) unsigned char smem[];
+    int* scol = reinterpret_cast<int*>(smem);  // columns, then value destinations
+    unsigned char* shdr = smem + 4 * (size_t)sv;
     const int lane = threadIdx.x;
-    const int t = lane % R;
     const int64_t br0 = (int64_t)blockIdx.x * BR;
-    const int64_t row0 = br0 * R;
-    const int64_t brE = br0 + BR < nbr ? br0 + BR : nbr;
-    const int64_t rowE = stage_rowptr(rowptr, row0, nrows, srp, lane);
-    const int v0 = srp[0];
-    const int nv = srp[rowE - row0] - v0;
-    const int64_t blk0 = bptr[br0];
-    const int64_t blkE = bptr[brE] < nblocks ? bptr[brE] : nblocks;
-    const int64_t g0 = blk0 * H, g1 = blkE * H;  // the group's header bytes
-    const int align = (int)(g0 & 3);
-    const bool staged = nv <= kStageValues && (g1 - g0) + align <= kStageHdrBytes;
-    const int64_t row = row0 + lane;
-    const int rb = srp[(row < nrows ? row : nrows) - row0];
-    const int re = srp[(row + 1 < nrows ? row + 1 : nrows) - row0];
     const int64_t br = br0 + lane / R;
+    // row pointers and block-row pointers, loaded together
+    const GroupRows g = group_rows(rowptr, br0 * R, nrows, lane);
+    const int b_lo = bptr[br < nbr ? br : nbr];
+    const int b_hi = bptr[br + 1 < nbr ? br + 1 : nbr];
+    const int64_t blk0 = __shfl(b_lo, 0, 64);
+    const int64_t blkE = (int64_t)__shfl(b_hi, 63, 64) < nblocks ? (int64_t)__shfl(b_hi, 63, 64) : nblocks;
     int nblk = 0, bo = 0;  // blocks of the lane's block-row, its first block's index in the group
     if (br < nbr) {
-        const int64_t b_beg = bptr[br];
-        const int64_t b_end = bptr[br + 1] < nblocks ? bptr[br + 1] : nblocks;
-        nblk = (int)(b_end > b_beg ? b_end - b_beg : 0);
-        bo = (int)(b_beg - blk0);
+        const int64_t b_end = b_hi < nblocks ? b_hi : nblocks;
+        nblk = (int)(b_end > b_lo ? b_end - b_lo : 0);
+        bo = (int)(b_lo - blk0);
     }
+    const int64_t g0 = blk0 * H, g1 = (blkE > blk0 ? blkE : blk0) * H;  // the group's header bytes
+    const int align = (int)(g0 & 3);
+    const int v0 = g.v0, nv = g.nv;
+    const bool staged = nv <= sv && (g1 - g0) + align <= shb;
     if (!staged) {
         // too large for the stage: global columns, direct writes
-        walk_group<R, VS>(rb, re, col, 0, [&](int k, int start, int tt, int first, int n, unsigned mask, int src0) {
+        walk_group<R, VS>(g.rb, g.re, col, 0, [&](int k, int start, int tt, int first, int n, unsigned mask, int src0) {
             if (k >= nblk) return;  // inconsistent pointers: never write past the block-row
             unsigned char* h = hdr + (blk0 + bo + k) * H;
             if (tt == 0) __builtin_memcpy(h, &start, 4);
@@ -237,17 +340,21 @@ __global__ __launch_bounds__(64) void k_convert_fill(int64_t nrows, int64_t nbr,
         });
         return;
     }
-    stage_words(col + v0, scol, nv, lane);
-    wave_sync();
-    walk_group<R, VS>(rb, re, scol, v0, [&](int k, int start, int tt, int first, int n, unsigned mask, int src0) {
+    if (nv == 0) return;
+    // the group's first 64 kPrefetch columns and values in one round trip
+    int c[kPrefetch];
+    W v[kPrefetch];
+    prefetch_words(col + v0, nv, lane, c);
+    prefetch_words(vin + v0, nv, lane, v);
+    stage_columns(col + v0, scol, nv, lane, c);
+    walk_group<R, VS>(g.rb, g.re, scol, v0, [&](int k, int start, int tt, int first, int n, unsigned mask, int src0) {
         if (k >= nblk) return;
         unsigned char* h = shdr + align + (bo + k) * H;
         if (tt == 0) __builtin_memcpy(h, &start, 4);
         __builtin_memcpy(h + 4 + tt * MB, &mask, MB);
-        for (int j = 0; j < n; ++j) dmap[src0 - v0 + j] = (unsigned short)(first - v0 + j);
+        for (int j = 0; j < n; ++j) scol[src0 - v0 + j] = first - v0 + j;
     });
     wave_sync();
-    (void)t;
     // headers: whole dwords of the destination as dword stores, the partial
     // first and last dwords byte by byte (their other bytes are another
     // group's)
@@ -266,42 +373,54 @@ __global__ __launch_bounds__(64) void k_convert_fill(int64_t nrows, int64_t nbr,
             }
         }
     }
-    // values: read in CSR order (8 loads in flight per lane), written to
-    // their block-order positions
+    // values: the prefetched ones from registers, the rest read in CSR order
+    // (8 loads in flight per lane); each written to its block-order position
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) {
+        const int i = lane + 64 * u;
+        if (i < nv) vout[v0 + scol[i]] = v[u];
+    }
     {
         constexpr int U = 8;
-        for (int i0 = 0; i0 < nv; i0 += 64 * U) {
-            W v[U];
+        for (int i0 = 64 * kPrefetch; i0 < nv; i0 += 64 * U) {
+            W w[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int i = i0 + lane + 64 * u;
-                v[u] = i < nv ? vin[v0 + i] : W(0);
+                w[u] = i < nv ? vin[v0 + i] : W(0);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int i = i0 + lane + 64 * u;
-                if (i < nv) vout[v0 + dmap[i]] = v[u];
+                if (i < nv) vout[v0 + scol[i]] = w[u];
             }
         }
     }
 }
 
 template <int R, int VS>
-void launch_count(int64_t nrows, int64_t nbr, const int* rowptr, const int* col, int* out, int* err,
+void launch_count(int64_t nrows, int64_t nbr, const int* rowptr, const int* col, int* out, int* err, int64_t nnz,
                   hipStream_t s) {
-    const unsigned g = (unsigned)((nbr + group_blockrows<R>() - 1) / group_blockrows<R>());
-    hipLaunchKernelGGL((k_convert_count<R, VS>), dim3(g), dim3(64), 0, s, nrows, nbr, rowptr, col, out, err);
+    const int64_t groups = (nbr + group_blockrows<R>() - 1) / group_blockrows<R>();
+    const int sv = stage_values(nnz, groups);
+    hipLaunchKernelGGL((k_convert_count<R, VS>), dim3((unsigned)groups), dim3(64), (size_t)sv * 4, s, nrows, nbr,
+                       rowptr, col, out, err, sv);
 }
 
 template <class W, int VS, int MB>
 int launch_fill_r(int r, int64_t nrows, int64_t nbr, const int* rowptr, const int* col, const void* vin,
-                  const int* bptr, int64_t nblocks, unsigned char* hdr, void* vout, hipStream_t s) {
+                  const int* bptr, int64_t nblocks, int64_t nnz, unsigned char* hdr, void* vout, hipStream_t s) {
     const W* vi = reinterpret_cast<const W*>(vin);
     W* vo = reinterpret_cast<W*>(vout);
 #define SPC5_FILL_LAUNCH(R)                                                                                    \
-    hipLaunchKernelGGL((k_convert_fill<W, R, VS, MB>), dim3((unsigned)((nbr + group_blockrows<R>() - 1) /        \
-                                                                       group_blockrows<R>())),                  \
-                       dim3(64), 0, s, nrows, nbr, rowptr, col, vi, bptr, nblocks, hdr, vo)
+    do {                                                                                                       \
+        const int64_t groups = (nbr + group_blockrows<R>() - 1) / group_blockrows<R>();                        \
+        const int sv = stage_values(nnz, groups);                                                              \
+        const int shb = stage_header_bytes(nblocks, 4 + R * MB, groups);                                       \
+        const size_t shm = (size_t)sv * 4 + (size_t)shb + 16;                                                  \
+        hipLaunchKernelGGL((k_convert_fill<W, R, VS, MB>), dim3((unsigned)groups), dim3(64), shm, s, nrows, nbr, \
+                           rowptr, col, vi, bptr, nblocks, hdr, vo, sv, shb);                                  \
+    } while (0)
     switch (r) {
     case 1: SPC5_FILL_LAUNCH(1); break;
     case 2: SPC5_FILL_LAUNCH(2); break;
@@ -316,10 +435,12 @@ int launch_fill_r(int r, int64_t nrows, int64_t nbr, const int* rowptr, const in
 
 template <class W>
 int launch_fill_w(int vs, int mb, int r, int64_t nrows, int64_t nbr, const int* rowptr, const int* col,
-                  const void* vin, const int* bptr, int64_t nblocks, unsigned char* hdr, void* vout, hipStream_t s) {
-    if (vs == 8 && mb == 1) return launch_fill_r<W, 8, 1>(r, nrows, nbr, rowptr, col, vin, bptr, nblocks, hdr, vout, s);
+                  const void* vin, const int* bptr, int64_t nblocks, int64_t nnz, unsigned char* hdr, void* vout,
+                  hipStream_t s) {
+    if (vs == 8 && mb == 1)
+        return launch_fill_r<W, 8, 1>(r, nrows, nbr, rowptr, col, vin, bptr, nblocks, nnz, hdr, vout, s);
     if (vs == 16 && mb == 2)
-        return launch_fill_r<W, 16, 2>(r, nrows, nbr, rowptr, col, vin, bptr, nblocks, hdr, vout, s);
+        return launch_fill_r<W, 16, 2>(r, nrows, nbr, rowptr, col, vin, bptr, nblocks, nnz, hdr, vout, s);
     set_error("(VS, mask bytes) must be (8, 1) or (16, 2)");
     return SPC5_HIP_ERR_INVALID;
 }
@@ -394,36 +515,33 @@ int convert_count_impl(int vs, int r, int64_t nrows, const int* d_rowptr, const 
         const unsigned g = (unsigned)((nrows + 1 + 255) / 256);
         hipLaunchKernelGGL(k_check_rowptr, dim3(g), dim3(256), 0, s, d_rowptr, nrows, ws.words);
         SPC5_HIP_TRY(hipGetLastError());
-        int h = 0;
-        int rc = read_words(ws.words, &h, 1, s);
-        if (rc) return rc;
-        // the count pass reads columns at the row pointers: only once they are valid
-        if (h) {
-            set_error("CSR row pointer must start at 0 and be non-decreasing");
-            return SPC5_HIP_ERR_INVALID;
-        }
     }
-    if (nbr > 0 && d_colidx == nullptr) {
-        // A matrix without entries may come without a column array (an
-        // empty tensor's storage); the count pass then reads no column.
-        int nnz_h = 0;
-        SPC5_HIP_TRY(hipMemcpyAsync(&nnz_h, d_rowptr + nrows, sizeof(int), hipMemcpyDeviceToHost, s));
-        SPC5_HIP_TRY(hipStreamSynchronize(s));
-        SPC5_CHECK_ARG(nnz_h == 0, "colidx is null");
+    // the flag and the value count (which sizes the count pass's stage)
+    int bad_rowptr = 0, nnz = 0;
+    SPC5_HIP_TRY(hipMemcpyAsync(&bad_rowptr, ws.words, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPC5_HIP_TRY(hipMemcpyAsync(&nnz, d_rowptr + nrows, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPC5_HIP_TRY(hipStreamSynchronize(s));
+    // the count pass reads columns at the row pointers: only once they are valid
+    if (bad_rowptr) {
+        set_error("CSR row pointer must start at 0 and be non-decreasing");
+        return SPC5_HIP_ERR_INVALID;
     }
+    // A matrix without entries may come without a column array (an empty
+    // tensor's storage); the count pass then reads no column.
+    SPC5_CHECK_ARG(nbr == 0 || d_colidx != nullptr || nnz == 0, "colidx is null");
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, d_rowsSize_out + nbr, (int64_t)1, 0);
     SPC5_HIP_TRY(hipGetLastError());
     if (nbr > 0) {
         const int key = r * 100 + vs;
         switch (key) {
-        case 108: launch_count<1, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
-        case 208: launch_count<2, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
-        case 408: launch_count<4, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
-        case 808: launch_count<8, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
-        case 116: launch_count<1, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
-        case 216: launch_count<2, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
-        case 416: launch_count<4, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
-        case 816: launch_count<8, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, s); break;
+        case 108: launch_count<1, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, nnz, s); break;
+        case 208: launch_count<2, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, nnz, s); break;
+        case 408: launch_count<4, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, nnz, s); break;
+        case 808: launch_count<8, 8>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, nnz, s); break;
+        case 116: launch_count<1, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, nnz, s); break;
+        case 216: launch_count<2, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, nnz, s); break;
+        case 416: launch_count<4, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, nnz, s); break;
+        case 816: launch_count<8, 16>(nrows, nbr, d_rowptr, d_colidx, d_rowsSize_out, ws.words + 1, nnz, s); break;
         }
         SPC5_HIP_TRY(hipGetLastError());
     }
@@ -501,8 +619,9 @@ int spc5_hip_convert_fill(int value_bytes, int vs, int mask_bytes, int r, int64_
     hipStream_t s = as_stream(stream);
     const int64_t nbr = (nrows + r - 1) / r;
     if (nbr == 0) return SPC5_HIP_OK;
-    int nb = 0;
+    int nb = 0, nnz = 0;
     SPC5_HIP_TRY(hipMemcpyAsync(&nb, d_rowsSize + nbr, sizeof(int), hipMemcpyDeviceToHost, s));
+    SPC5_HIP_TRY(hipMemcpyAsync(&nnz, d_rowptr + nrows, sizeof(int), hipMemcpyDeviceToHost, s));
     SPC5_HIP_TRY(hipStreamSynchronize(s));
     if (nb == 0) return SPC5_HIP_OK;
     SPC5_CHECK_ARG(d_headers_out != nullptr && d_values_out != nullptr && d_values_in != nullptr,
@@ -510,13 +629,13 @@ int spc5_hip_convert_fill(int value_bytes, int vs, int mask_bytes, int r, int64_
     switch (value_bytes) {
     case 8:
         return launch_fill_w<unsigned long long>(vs, mask_bytes, r, nrows, nbr, d_rowptr, d_colidx, d_values_in,
-                                                 d_rowsSize, nb, d_headers_out, d_values_out, s);
+                                                 d_rowsSize, nb, nnz, d_headers_out, d_values_out, s);
     case 4:
         return launch_fill_w<unsigned int>(vs, mask_bytes, r, nrows, nbr, d_rowptr, d_colidx, d_values_in,
-                                           d_rowsSize, nb, d_headers_out, d_values_out, s);
+                                           d_rowsSize, nb, nnz, d_headers_out, d_values_out, s);
     default:
         return launch_fill_w<unsigned short>(vs, mask_bytes, r, nrows, nbr, d_rowptr, d_colidx, d_values_in,
-                                             d_rowsSize, nb, d_headers_out, d_values_out, s);
+                                             d_rowsSize, nb, nnz, d_headers_out, d_values_out, s);
     }
 }
 

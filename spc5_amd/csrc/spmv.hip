@@ -75,31 +75,58 @@ __global__ void k_check_bptr(const int* __restrict__ bptr, int64_t nbr, int64_t 
 }
 
 // Values per block-row (popcount of every mask) and a column range check.
+// One wave per 64 block-rows (the pointers already checked): the lanes read
+// the group's headers in block order, coalesced, find each block's block-row
+// by a binary search of the 65 group pointers in LDS and add its popcount to
+// that block-row's LDS counter.
 template <int R, int MB, int VS>
-__global__ void k_blockrow_values(const int* __restrict__ bptr, const unsigned char* __restrict__ hdr,
-                                  int64_t nbr, int64_t ncols, int* __restrict__ cnt_out, int* err) {
+__global__ __launch_bounds__(256) void k_blockrow_values(const int* __restrict__ bptr,
+                                                         const unsigned char* __restrict__ hdr, int64_t nbr,
+                                                         int64_t ncols, int* __restrict__ cnt_out, int* err) {
     constexpr int H = 4 + R * MB;
     using MW = MaskWords<R * MB>;
-    const int64_t br = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (br >= nbr) return;
-    int c = 0;
+    __shared__ int sptr[4][65];
+    __shared__ int scnt[4][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t br0 = ((int64_t)blockIdx.x * 4 + w) * 64;
+    if (br0 >= nbr) return;  // whole wave: nothing below synchronises across waves
+    int* sp = sptr[w];
+    int* sc = scnt[w];
+    {
+        const int64_t i = br0 + lane;
+        sp[lane] = bptr[i < nbr ? i : nbr];
+        if (lane == 0) sp[64] = bptr[br0 + 64 < nbr ? br0 + 64 : nbr];
+        sc[lane] = 0;
+    }
+    wave_sync();
+    const int b0 = sp[0], bE = sp[64];
     bool bad = false;
-    for (int b = bptr[br]; b < bptr[br + 1]; ++b) {
+    for (int b = b0 + lane; b < bE; b += 64) {
         const unsigned char* h = hdr + (int64_t)b * H;
         const int cs = load_i32_unaligned(h);
         unsigned long long mw[MW::kWords];
         MW::load(h + 4, mw);
+        int c = 0;
         unsigned rowor = 0;
         for (int k = 0; k < MW::kWords; ++k) {
             c += __popcll(mw[k]);
-            for (int s = 0; s < 64 / VS; ++s) rowor |= (unsigned)((mw[k] >> (s * VS)) & ((1ull << VS) - 1ull));
+            for (int q = 0; q < 64 / VS; ++q) rowor |= (unsigned)((mw[k] >> (q * VS)) & ((1ull << VS) - 1ull));
         }
         if (rowor != 0) {
             const int hi = 31 - __builtin_clz(rowor);
             bad |= cs < 0 || (int64_t)cs + hi >= ncols;
         }
+        // the block's block-row: the largest i with sp[i] <= b (sp[0] <= b < sp[64])
+        int lo = 0, up = 64;
+        while (up - lo > 1) {
+            const int mid = (lo + up) >> 1;
+            if (sp[mid] <= b) lo = mid;
+            else up = mid;
+        }
+        atomicAdd(&sc[lo], c);
     }
-    cnt_out[br] = c;
+    wave_sync();
+    if (br0 + lane < nbr) cnt_out[br0 + lane] = sc[lane];
     if (bad) atomicOr(err, 2);
 }
 
@@ -133,7 +160,7 @@ __global__ void k_tile_scatter(const int* __restrict__ valptr, int64_t nbr, int 
 
 template <int R, int MB, int VS>
 void launch_blockrow_values(const spc5_hip_mat& m, int* cnt, int* err, hipStream_t s) {
-    const unsigned g = (unsigned)((m.nbr + 255) / 256);
+    const unsigned g = (unsigned)((m.nbr + 255) / 256);  // 4 waves of 64 block-rows per workgroup
     hipLaunchKernelGGL((k_blockrow_values<R, MB, VS>), dim3(g), dim3(256), 0, s, m.bptr, m.hdr, m.nbr, m.ncols, cnt,
                        err);
 }
