@@ -461,6 +461,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-max-rows", type=int, default=0)
     ap.add_argument("--no-copy", action="store_true")
+    ap.add_argument("--convert-reps", type=int, default=3, help="conversions timed per format")
     ap.add_argument("--no-ceiling", action="store_true", help="skip the streaming-probe ceiling")
     ap.add_argument("--no-tune-tiles", action="store_true", help="default tile sizes (no tile autotuning)")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 variant of the SpMV matrix")
@@ -500,10 +501,22 @@ def main():
     formats = {}
     best = None
     for r in [int(s) for s in args.formats.split(",")]:
-        tc = time.perf_counter()
-        m = DeviceMatrix.from_csr(d_rp, d_col, d_val, r)
-        torch.cuda.synchronize()
-        conv_s = time.perf_counter() - tc
+        # conversion on the GPU (count pass, scan, fill pass, tile plan), timed
+        # on the host around a synchronised call; the first call carries
+        # one-time set-up, the later ones are the steady cost
+        conv = []
+        for rep in range(max(1, args.convert_reps)):
+            if rep:
+                del m
+            tc = time.perf_counter()
+            m = DeviceMatrix.from_csr(d_rp, d_col, d_val, r)
+            torch.cuda.synchronize()
+            conv.append(time.perf_counter() - tc)
+        conv_s = conv[0]
+        conv_ms = 1e3 * (min(conv[1:]) if len(conv) > 1 else conv[0])
+        # bytes a one-pass conversion must move: rowptr, columns and values
+        # read, block-row pointers, headers and values written
+        conv_bytes = 4 * (n + 1) + (4 + 8) * nnz + 4 * ((n + r - 1) // r + 1) + (4 + r) * m.nblocks + 8 * nnz
         # the tile size is the autotuner's per-matrix choice (timed candidates)
         tile, tile_times = autotune.tune_tiles(m) if not args.no_tune_tiles else (0, {})
         ms, wall = time_spmv(m, x, y, args.steps, args.warmup)
@@ -516,9 +529,13 @@ def main():
                                   "tile_candidates_ms": {str(k): round(v * 1e3, 5) for k, v in tile_times.items()},
                                   "bytes_per_spmv": B, "ms": round(ms, 5), "gflops": round(gflops, 2),
                                   "gbs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
-                                  "convert_s": round(conv_s, 3), "timed_wall_s": round(wall, 3)}
+                                  "convert_s": round(conv_s, 3), "convert_ms": round(conv_ms, 3),
+                                  "convert_bytes": conv_bytes,
+                                  "convert_gbs": round(conv_bytes / (conv_ms * 1e-3) / 1e9, 1),
+                                  "timed_wall_s": round(wall, 3)}
         log(f"[bench] beta({r},8): {ms:.4f} ms  {gflops:.1f} GFLOP/s  {gbs:.0f} GB/s  "
             f"({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)  nb={m.nblocks} tiles={info['ntiles']} "
+            f"convert {conv_ms:.2f} ms (first {1e3 * conv_s:.2f}) "
             f"(tile size {m.tile_target()} values; candidates "
             f"{', '.join(f'{k}: {v * 1e3:.4f}' for k, v in tile_times.items())})")
         if best is None or ms < formats[best]["ms"]:

@@ -50,6 +50,7 @@ constexpr int kGroupRows = 64;
 constexpr int kPrefetch = 32;             // words per lane loaded at once: 2048 per group
 constexpr int kMaxStageValues = 8192;     // 32 KiB of staged columns
 constexpr int kMaxStageHdrBytes = 16384;  // staged header bytes
+constexpr int kStagePad = 32;             // words after the staged columns read by the walk's windows
 
 template <int R>
 constexpr int group_blockrows() { return kGroupRows / R; }
@@ -120,18 +121,6 @@ __device__ __forceinline__ int group_excl_scan(int v, int t) {
     return incl - v;
 }
 
-// One row's cursor.  `col` is indexed by global value position minus
-// `base` (0 for the global array, the group's first value for the stage).
-struct RowCursor {
-    int cur, end, head;  // head: column at the cursor, INT_MAX once consumed
-    template <class C>
-    __device__ __forceinline__ void init(int b, int e, C col, int base) {
-        cur = b;
-        end = e;
-        head = b < e ? col[b - base] : INT_MAX;
-    }
-};
-
 // Walks the block-rows of a group; for each block of a lane's block-row
 // calls on_block(k, start, t, first, n, mask, src0) on every lane of the
 // group: k is the block's index in its block-row, t the lane's row in the
@@ -140,35 +129,60 @@ struct RowCursor {
 // at source positions [src0, src0 + n), whose columns the walk never reads
 // again.  Returns whether a lane saw columns that break the format's
 // precondition (non-negative, strictly increasing).
-template <int R, int VS, class C, class F>
+//
+// `col` is indexed by global value position minus `base` (0 for the global
+// array, the group's first value for the stage).  Per block a lane reads the
+// window of its next VS + 1 columns at once: its values in the block are the
+// window's leading run below start + VS (at most VS, the columns being
+// strictly increasing), the entry after them is the next head, and adjacent
+// window entries check the precondition.  PADDED: `col` may be read up to
+// VS + 1 entries past the group's last value (the stage's pad), so the window
+// is read without clamping; otherwise each index is clamped to the row.
+template <int R, int VS, bool PADDED, class C, class F>
 __device__ __forceinline__ bool walk_group(int rb, int re, C col, int base, F on_block) {
     const int lane = threadIdx.x;
     const int t = lane % R;
-    RowCursor c;
-    c.init(rb, re, col, base);
-    bool bad = c.head < 0;
+    const int end = re;
+    int cur = rb;
+    int head = cur < end ? col[cur - base] : INT_MAX;  // INT_MAX once the row is consumed
+    bool bad = head < 0;
     // the block-row's first value: the row pointer of its row 0
     int vbase = __shfl(rb, lane - t, 64);
     int k = 0;
     for (;;) {
-        const int start = group_min<R>(c.head);
+        const int start = group_min<R>(head);
         if (!ballot64(start != INT_MAX)) break;  // every block-row of the wave is done
         if (start != INT_MAX) {
-            const int64_t lim = (int64_t)start + VS;
-            const int src0 = c.cur;
+            int w[VS + 1];
+            if constexpr (PADDED) {
+                const auto p = col + (cur - base);
+#pragma unroll
+                for (int j = 0; j <= VS; ++j) w[j] = p[j];
+            } else {
+                const int last = end - 1 > cur ? end - 1 : cur;
+#pragma unroll
+                for (int j = 0; j <= VS; ++j) w[j] = cur < end ? col[(cur + j < last ? cur + j : last) - base] : 0;
+            }
             unsigned mask = 0;
             int n = 0;
-            while (c.cur < c.end && c.head < lim) {
-                mask |= 1u << (unsigned)(c.head - start);
-                const int prev = c.head;
-                ++c.cur;
-                ++n;
-                c.head = c.cur < c.end ? col[c.cur - base] : INT_MAX;
-                bad |= c.cur < c.end && c.head <= prev;
+            bool alive = true;
+#pragma unroll
+            for (int j = 0; j < VS; ++j) {
+                const unsigned d = (unsigned)w[j] - (unsigned)start;
+                alive = alive && cur + j < end && d < (unsigned)VS;
+                mask |= alive ? 1u << (d & 31u) : 0u;
+                n += alive ? 1 : 0;
             }
+#pragma unroll
+            for (int j = 1; j <= VS; ++j) bad |= cur + j < end && w[j] <= w[j - 1];
+            int next = w[0];
+#pragma unroll
+            for (int j = 1; j <= VS; ++j) next = n == j ? w[j] : next;
             const int before = group_excl_scan<R>(n, t);
-            on_block(k, start, t, vbase + before, n, mask, src0);
+            on_block(k, start, t, vbase + before, n, mask, cur);
             vbase += group_sum<R>(n);
+            cur += n;
+            head = cur < end ? next : INT_MAX;
             ++k;
         }
     }
@@ -266,7 +280,7 @@ __device__ __forceinline__ GroupRows group_rows(const int* __restrict__ rowptr, 
 
 // Blocks per block-row (the count pass).  A violation of the column
 // precondition sets bit 0 of *err; the fill pass is then never run.  Dynamic
-// LDS: sv staged column indices.
+// LDS: sv staged column indices and kStagePad words of pad.
 template <int R, int VS>
 __global__ __launch_bounds__(64) void k_convert_count(int64_t nrows, int64_t nbr, const int* __restrict__ rowptr,
                                                       const int* __restrict__ col, int* __restrict__ cnt_out,
@@ -286,9 +300,9 @@ __global__ __launch_bounds__(64) void k_convert_count(int64_t nrows, int64_t nbr
             prefetch_words(col + g.v0, g.nv, lane, c);
             stage_columns(col + g.v0, scol, g.nv, lane, c);
         }
-        bad = walk_group<R, VS>(g.rb, g.re, scol, g.v0, on_block);
+        bad = walk_group<R, VS, true>(g.rb, g.re, scol, g.v0, on_block);
     } else {
-        bad = walk_group<R, VS>(g.rb, g.re, col, 0, on_block);
+        bad = walk_group<R, VS, false>(g.rb, g.re, col, 0, on_block);
     }
     const int64_t br = br0 + lane / R;
     if (lane % R == 0 && br < nbr) cnt_out[br] = nb;
@@ -296,9 +310,10 @@ __global__ __launch_bounds__(64) void k_convert_count(int64_t nrows, int64_t nbr
 }
 
 // Headers and values of the groups' blocks.  Dynamic LDS: sv staged column
-// indices, overwritten in place by the destination of each value once the
-// walk has consumed its column, then shb + 4 header bytes (the group's
-// headers, offset by the byte alignment of their destination).
+// indices (and kStagePad words of pad), overwritten in place by the
+// destination of each value once the walk has consumed its column, then
+// shb + 4 header bytes (the group's headers, offset by the byte alignment of
+// their destination).
 template <class W, int R, int VS, int MB>
 __global__ __launch_bounds__(64) void k_convert_fill(int64_t nrows, int64_t nbr, const int* __restrict__ rowptr,
                                                      const int* __restrict__ col, const W* __restrict__ vin,
@@ -309,7 +324,7 @@ __global__ __launch_bounds__(64) void k_convert_fill(int64_t nrows, int64_t nbr,
     constexpr int H = 4 + R * MB;
     extern __shared__ __align__(16) unsigned char smem[];
     int* scol = reinterpret_cast<int*>(smem);  // columns, then value destinations
-    unsigned char* shdr = smem + 4 * (size_t)sv;
+    unsigned char* shdr = smem + 4 * (size_t)(sv + kStagePad);
     const int lane = threadIdx.x;
     const int64_t br0 = (int64_t)blockIdx.x * BR;
     const int64_t br = br0 + lane / R;
@@ -331,7 +346,8 @@ __global__ __launch_bounds__(64) void k_convert_fill(int64_t nrows, int64_t nbr,
     const bool staged = nv <= sv && (g1 - g0) + align <= shb;
     if (!staged) {
         // too large for the stage: global columns, direct writes
-        walk_group<R, VS>(g.rb, g.re, col, 0, [&](int k, int start, int tt, int first, int n, unsigned mask, int src0) {
+        walk_group<R, VS, false>(g.rb, g.re, col, 0, [&](int k, int start, int tt, int first, int n, unsigned mask,
+                                                          int src0) {
             if (k >= nblk) return;  // inconsistent pointers: never write past the block-row
             unsigned char* h = hdr + (blk0 + bo + k) * H;
             if (tt == 0) __builtin_memcpy(h, &start, 4);
@@ -347,12 +363,15 @@ __global__ __launch_bounds__(64) void k_convert_fill(int64_t nrows, int64_t nbr,
     prefetch_words(col + v0, nv, lane, c);
     prefetch_words(vin + v0, nv, lane, v);
     stage_columns(col + v0, scol, nv, lane, c);
-    walk_group<R, VS>(g.rb, g.re, scol, v0, [&](int k, int start, int tt, int first, int n, unsigned mask, int src0) {
+    walk_group<R, VS, true>(g.rb, g.re, scol, v0, [&](int k, int start, int tt, int first, int n, unsigned mask,
+                                                      int src0) {
         if (k >= nblk) return;
         unsigned char* h = shdr + align + (bo + k) * H;
         if (tt == 0) __builtin_memcpy(h, &start, 4);
         __builtin_memcpy(h + 4 + tt * MB, &mask, MB);
-        for (int j = 0; j < n; ++j) scol[src0 - v0 + j] = first - v0 + j;
+#pragma unroll
+        for (int j = 0; j < VS; ++j)
+            if (j < n) scol[src0 - v0 + j] = first - v0 + j;
     });
     wave_sync();
     // headers: whole dwords of the destination as dword stores, the partial
@@ -403,7 +422,8 @@ void launch_count(int64_t nrows, int64_t nbr, const int* rowptr, const int* col,
                   hipStream_t s) {
     const int64_t groups = (nbr + group_blockrows<R>() - 1) / group_blockrows<R>();
     const int sv = stage_values(nnz, groups);
-    hipLaunchKernelGGL((k_convert_count<R, VS>), dim3((unsigned)groups), dim3(64), (size_t)sv * 4, s, nrows, nbr,
+    hipLaunchKernelGGL((k_convert_count<R, VS>), dim3((unsigned)groups), dim3(64), (size_t)(sv + kStagePad) * 4, s,
+                       nrows, nbr,
                        rowptr, col, out, err, sv);
 }
 
@@ -417,7 +437,7 @@ int launch_fill_r(int r, int64_t nrows, int64_t nbr, const int* rowptr, const in
         const int64_t groups = (nbr + group_blockrows<R>() - 1) / group_blockrows<R>();                        \
         const int sv = stage_values(nnz, groups);                                                              \
         const int shb = stage_header_bytes(nblocks, 4 + R * MB, groups);                                       \
-        const size_t shm = (size_t)sv * 4 + (size_t)shb + 16;                                                  \
+        const size_t shm = (size_t)(sv + kStagePad) * 4 + (size_t)shb + 16;                                    \
         hipLaunchKernelGGL((k_convert_fill<W, R, VS, MB>), dim3((unsigned)groups), dim3(64), shm, s, nrows, nbr, \
                            rowptr, col, vi, bptr, nblocks, hdr, vo, sv, shb);                                  \
     } while (0)
