@@ -26,11 +26,9 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# The CPU baseline's OpenMP threads are bound one per core, as the
-# reference's job scripts do (jobs-plafrim.sh:18-19); set before any library
-# loads libgomp.
-os.environ.setdefault("OMP_PROC_BIND", "true")
-os.environ.setdefault("OMP_PLACES", "cores")
+# The host CPUs this process may use, read before any library starts an
+# OpenMP runtime (one that binds its first thread would shrink the set).
+HOST_CPUS = sorted(os.sched_getaffinity(0))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "SpMV GFLOP/s + HBM GB/s (% of peak); mixed vs FP64 solve time at 1e-10"
@@ -127,6 +125,30 @@ def stream_ceiling(read_bytes, write_bytes, tiles_kib=(8, 16, 32, 64), steps=20)
     return max(rates.values()), rates
 
 
+def access_split(m, x, steps=50):
+    """Times the library's access probe on a double beta(1,8) matrix: the
+    product's tiles and grid reading every header and value (the matrix
+    stream), then with x gathered at every value's column in the product's
+    order, then with those gathers folded into a 1024-entry window (all L1
+    hits).  Splits a product's time into streaming, gathering and the rest."""
+    import ctypes as C
+
+    import torch
+    from spc5_amd._lib import check, lib
+    out = torch.zeros(1, dtype=torch.float64, device="cuda")
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    res = {}
+    for mode, name in ((0, "matrix_stream_ms"), (1, "with_x_gathers_ms"), (2, "with_l1_gathers_ms")):
+        def run(mode=mode):
+            check(lib().spc5_hip_access_probe(m._h, mode, C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()),
+                                              stream), "access probe")
+        res[name] = round(_events_time(run, steps, 3) * 1e3, 5)
+    stream_bytes = 8 * m.nnz + 5 * m.nblocks
+    res["matrix_stream_bytes"] = stream_bytes
+    res["matrix_stream_gbs"] = round(stream_bytes / (res["matrix_stream_ms"] * 1e-3) / 1e9, 1)
+    return res
+
+
 def traffic_from_profiles(workload, fmt):
     p = os.path.join(ROOT, "profiles", f"pmc_spmv_{workload}_{fmt}.json")
     if not os.path.exists(p):
@@ -145,6 +167,29 @@ def cpu_model():
     except OSError:
         pass
     return None
+
+
+def cpu_baseline_in_child(args, r, timeout=900):
+    """Runs the CPU baseline in a fresh process (`--cpu-baseline-child`) on
+    the host CPUs this process started with, its OpenMP threads bound one per
+    core as the reference's job scripts do (jobs-plafrim.sh:18-19).  The
+    child touches no GPU; it rebuilds the same seeded workload."""
+    import subprocess
+    cpus = HOST_CPUS[: args.cpu_threads] if args.cpu_threads else HOST_CPUS
+    env = dict(os.environ, OMP_PROC_BIND="true", OMP_PLACES="cores", OMP_NUM_THREADS=str(len(cpus)))
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", "--workload", args.workload,
+           "--cpu-r", str(r), "--cpu-list", ",".join(str(c) for c in cpus)]
+    if args.n:
+        cmd += ["--n", str(args.n)]
+    if args.cpu_max_rows:
+        cmd += ["--cpu-max-rows", str(args.cpu_max_rows)]
+    try:
+        res = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return {"error": f"CPU baseline exceeded {timeout} s"}
+    if res.returncode != 0:
+        return {"error": f"CPU baseline child exited with {res.returncode}: {res.stderr.strip()[-400:]}"}
+    return json.loads(res.stdout.strip().splitlines()[-1])
 
 
 def cpu_baseline(rp, col, val, r, ncols, nthreads, nloops=16, nsamples=5, max_rows=None):
@@ -471,6 +516,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-count", type=int, default=2000)
     ap.add_argument("--only-extensions", action="store_true", help="skip the headline SpMV sections")
+    ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-r", type=int, default=1, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-list", default="", help=argparse.SUPPRESS)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -479,6 +527,16 @@ def main():
               file=sys.stderr)
         return 2
 
+    if args.cpu_baseline_child:
+        # the CPU baseline alone, in its own process: no GPU, no torch
+        cpus = [int(c) for c in args.cpu_list.split(",") if c] or HOST_CPUS
+        os.sched_setaffinity(0, cpus)
+        rp, col, val, _ = make_workload(args.workload, args.n)
+        out = cpu_baseline(rp, col, val, args.cpu_r, len(rp) - 1, len(cpus),
+                           max_rows=args.cpu_max_rows or None)
+        print(json.dumps(out), flush=True)
+        return 0
+
     import numpy as np
     import torch
     from spc5_amd import synth
@@ -486,6 +544,17 @@ def main():
 
     assert torch.cuda.is_available(), "needs a HIP device"
     t_all = time.perf_counter()
+    if args.only_extensions:
+        out = {"only_extensions": True}
+        if not args.no_solve:
+            out["solve"] = solve_benchmark(args.solve_n)
+        if not args.no_spmm:
+            out["spmm"] = spmm_benchmark(args.solve_n)
+        if not args.no_batch:
+            out["batch"] = batch_benchmark(args.batch_count)
+        log(f"[bench] total {time.perf_counter() - t_all:.1f}s")
+        print(json.dumps(out), flush=True)
+        return 0
     rp, col, val, desc = make_workload(args.workload, args.n)
     n = len(rp) - 1
     nnz = int(rp[-1])
@@ -538,6 +607,15 @@ def main():
             f"convert {conv_ms:.2f} ms (first {1e3 * conv_s:.2f}) "
             f"(tile size {m.tile_target()} values; candidates "
             f"{', '.join(f'{k}: {v * 1e3:.4f}' for k, v in tile_times.items())})")
+        if r == 1:
+            # where the product's time goes: the same tiles with the matrix
+            # stream only, with the x gathers, with L1-resident gathers
+            split = access_split(m, x)
+            split["product_ms"] = round(ms, 5)
+            formats[m.format_name]["access_split"] = split
+            log(f"[bench] beta(1,8) access split: matrix stream {split['matrix_stream_ms']:.4f} ms "
+                f"({split['matrix_stream_gbs']} GB/s), + x gathers {split['with_x_gathers_ms']:.4f} ms, "
+                f"+ gathers in L1 {split['with_l1_gathers_ms']:.4f} ms, product {ms:.4f} ms")
         if best is None or ms < formats[best]["ms"]:
             best = m.format_name
         del m
@@ -597,6 +675,7 @@ def main():
                      "ceiling_measured": ceiling["gbs"] if ceiling else None,
                      "frac_of_ceiling": round(bf["gbs"] / ceiling["gbs"], 4) if ceiling else None},
         "stream_ceiling": ceiling,
+        "access_split": formats[best].get("access_split"),
         "formats": formats,
         "formats_f32": formats_f32,
         "copy_gbs_measured": copy_gbs,
@@ -610,10 +689,11 @@ def main():
     if not args.no_batch:
         out["batch"] = batch_benchmark(args.batch_count)
     if not args.no_cpu:
-        log("[bench] CPU baseline (reference OpenMP path)...")
-        th = args.cpu_threads or len(os.sched_getaffinity(0)) or 1
-        out["cpu_baseline"] = cpu_baseline(rp, col, val, bf["r"], ncols, th,
-                                           max_rows=args.cpu_max_rows or None)
+        log("[bench] CPU baseline (reference OpenMP path, child process)...")
+        out["cpu_baseline"] = cpu_baseline_in_child(args, bf["r"])
+        cb = out["cpu_baseline"]
+        log(f"[bench] CPU baseline: {cb.get('value')} GFLOP/s on {cb.get('cores')} cores "
+            f"({cb.get('samples_gflops') or cb.get('error')})")
     else:
         out["cpu_baseline"] = None
     log(f"[bench] total {time.perf_counter() - t_all:.1f}s")

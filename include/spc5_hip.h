@@ -333,6 +333,17 @@ int spc5_hip_synth_banded_fill(int64_t n, int64_t bw, uint64_t thr, uint64_t see
 int spc5_hip_stream_probe(const void* d_src, int64_t read_bytes, void* d_dst, int64_t write_bytes,
                           int64_t tile_bytes, void* stream);
 
+/* Memory-access probe of a double beta(1,8) product (no counterpart in the
+ * reference; a measurement aid).  It walks the handle's product tiles on the
+ * product's grid and reads every header and value as the product does, and
+ * then: mode 0, nothing more (the matrix stream alone); mode 1, x at every
+ * value's column in the product's gather order; mode 2, the same gathers
+ * folded into a 1024-entry window of x (all L1 hits; needs ncols >= 1024).
+ * No row sums, no y: timing the three modes splits a product's time into
+ * streaming, gathering and the rest.  d_out must hold 8 bytes (written only
+ * on an impossible sum). */
+int spc5_hip_access_probe(spc5_hip_mat_t h, int mode, const void* d_x, void* d_out, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* 3. Host-pointer helpers (used by include/spc5_hip.hpp)                 */
 /* ---------------------------------------------------------------------- */

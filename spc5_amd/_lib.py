@@ -62,6 +62,7 @@ SIGNATURES = {
     "spc5_hip_synth_banded_count": (_I, [_I64, _I64, C.c_uint64, C.c_uint64, _P, C.POINTER(_I64), _P]),
     "spc5_hip_synth_banded_fill": (_I, [_I64, _I64, C.c_uint64, C.c_uint64, _P, _P, _P, _P]),
     "spc5_hip_stream_probe": (_I, [_P, _I64, _P, _I64, _I64, _P]),
+    "spc5_hip_access_probe": (_I, [_P, _I, _P, _P, _P]),
     "spc5_hip_batch_create": (_I, [_I, _P, _P, _P, _I, C.POINTER(_P)]),
     "spc5_hip_batch_spmv": (_I, [_P, _P]),
     "spc5_hip_batch_spmv_direct": (_I, [_P, _P]),

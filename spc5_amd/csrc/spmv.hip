@@ -60,6 +60,87 @@ k_spmv_tiles(
                                           tile_val[t + 1], nrows, ncols, e);
 }
 
+// ---- access probe ----------------------------------------------------------
+
+// Memory-access probe over a double beta(1,8) matrix (a measurement aid, not
+// a product): the product's tiles on the product's grid, one wave per tile,
+// batches of 64 blocks (one per lane, at most 512 values).  Every block's
+// header (fetched one batch ahead) and the batch's values (coalesced,
+// non-temporal) are read, and then
+//   MODE 0: nothing more -- the matrix stream alone;
+//   MODE 1: x at every value's column, expanded through LDS in value order as
+//           the product gathers it;
+//   MODE 2: as 1 with each column folded into a 1024-entry window of x at the
+//           tile's first column: the same gathers, all L1 hits.
+// There are no row sums and no y; the sum of the products reaches memory only
+// on an impossible value, which keeps every load live.
+template <int MODE>
+__global__ __launch_bounds__(kWavesPerGroup * 64) void k_access_probe(
+    const int* __restrict__ bptr, const unsigned char* __restrict__ hdr, const double* __restrict__ val,
+    const double* __restrict__ x, const int* __restrict__ tile_br, const int* __restrict__ tile_val, int64_t ntiles,
+    int64_t ncols, double* __restrict__ out) {
+    constexpr int H = 5;  // int32 first column + one mask byte
+    __shared__ int scol[kWavesPerGroup][512];
+    const int w = threadIdx.x >> 6;
+    const int64_t t = xcd_tile(w);
+    if (t >= ntiles) return;
+    const int lane = threadIdx.x & 63;
+    int* sc = scol[w];
+    const int b0 = bptr[tile_br[t]], b1 = bptr[tile_br[t + 1]];
+    int64_t v = tile_val[t];
+    double acc = 0.0;
+    int cs = 0;
+    unsigned m = 0;
+    if (b0 + lane < b1) {
+        cs = load_i32_unaligned(hdr + (int64_t)(b0 + lane) * H);
+        m = hdr[(int64_t)(b0 + lane) * H + 4];
+    }
+    // the folded window (MODE 2): 1024 columns from the tile's first column
+    const int64_t wbase = ncols >= 1024 ? (__shfl(cs, 0, 64) < ncols - 1024 ? __shfl(cs, 0, 64) : ncols - 1024) : 0;
+    for (int b = b0; b < b1; b += 64) {
+        const int cnt = __popc(m);
+        const int incl = wave_incl_scan(cnt);
+        const int total = __builtin_amdgcn_readlane(incl, 63);
+        double a[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int q = lane + 64 * k;
+            a[k] = q < total ? __builtin_nontemporal_load(val + v + q) : 0.0;
+        }
+        // next batch's header
+        int ncs = 0;
+        unsigned nm = 0;
+        if (b + 64 + lane < b1) {
+            ncs = load_i32_unaligned(hdr + (int64_t)(b + 64 + lane) * H);
+            nm = hdr[(int64_t)(b + 64 + lane) * H + 4];
+        }
+        if constexpr (MODE == 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc += a[k];
+            acc += (double)cs;
+        } else {
+            int p = incl - cnt;
+#pragma unroll
+            for (int bit = 0; bit < 8; ++bit)
+                if ((m >> bit) & 1u) sc[p++] = MODE == 1 ? cs + bit : (int)(wbase + ((cs + bit) & 1023));
+            wave_sync();
+            double xv[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int q = lane + 64 * k;
+                xv[k] = q < total ? x[sc[q]] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc += a[k] * xv[k];
+            wave_sync();
+        }
+        v += total;
+        cs = ncs;
+        m = nm;
+    }
+    if (acc == 1.2345e300) out[0] = acc;
+}
+
 // ---- plan kernels --------------------------------------------------------
 
 // Checks the block-row pointers: 0-based, non-decreasing, ending at nblocks.
@@ -431,6 +512,40 @@ int spc5_hip_mat_set_option(spc5_hip_mat_t h, int option, int64_t value) {
         set_error("unknown option");
         return SPC5_HIP_ERR_INVALID;
     }
+}
+
+int spc5_hip_access_probe(spc5_hip_mat_t h, int mode, const void* d_x, void* d_out, void* stream) {
+    SPC5_CHECK_ARG(h != nullptr && d_x != nullptr && d_out != nullptr, "null argument");
+    SPC5_CHECK_ARG(h->value_type == SPC5_HIP_F64 && h->r == 1 && h->vs == 8 && h->mb == 1,
+                   "the access probe takes a double beta(1,8) matrix");
+    SPC5_CHECK_ARG(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
+    SPC5_CHECK_ARG(mode != 2 || h->ncols >= 1024, "mode 2 needs at least 1024 columns");
+    const spc5_hip_mat& m = *h;
+    if (m.plan.ntiles == 0) return SPC5_HIP_OK;
+    const int64_t groups = (m.plan.ntiles + kWavesPerGroup - 1) / kWavesPerGroup;
+    const unsigned grid = (unsigned)((groups + 7) / 8 * 8);
+    const double* xv = reinterpret_cast<const double*>(d_x);
+    const double* vv = reinterpret_cast<const double*>(m.values);
+    double* o = reinterpret_cast<double*>(d_out);
+    const int* tb = m.plan.tile_br->as<int>();
+    const int* tv = m.plan.tile_val->as<int>();
+    hipStream_t s = as_stream(stream);
+    switch (mode) {
+    case 0:
+        hipLaunchKernelGGL(k_access_probe<0>, dim3(grid), dim3(kWavesPerGroup * 64), 0, s, m.bptr, m.hdr, vv, xv, tb, tv,
+                           m.plan.ntiles, m.ncols, o);
+        break;
+    case 1:
+        hipLaunchKernelGGL(k_access_probe<1>, dim3(grid), dim3(kWavesPerGroup * 64), 0, s, m.bptr, m.hdr, vv, xv, tb, tv,
+                           m.plan.ntiles, m.ncols, o);
+        break;
+    default:
+        hipLaunchKernelGGL(k_access_probe<2>, dim3(grid), dim3(kWavesPerGroup * 64), 0, s, m.bptr, m.hdr, vv, xv, tb, tv,
+                           m.plan.ntiles, m.ncols, o);
+        break;
+    }
+    SPC5_HIP_TRY(hipGetLastError());
+    return SPC5_HIP_OK;
 }
 
 int spc5_hip_spmv(spc5_hip_mat_t h, const void* d_x, void* d_y, void* stream) {
