@@ -474,12 +474,15 @@ def batch_benchmark(count=2000, fill=0.3, steps=10, warmup=2):
         info = infos[srt]
         tiles_hist = {}
         for m in ms:
-            tiles_hist[m.tile_target()] = tiles_hist.get(m.tile_target(), 0) + 1
+            # tile sizes by power-of-two range: "512-1023" holds 512 <= size < 1024
+            lo = 1 << (max(1, m.tile_target()).bit_length() - 1)
+            key = f"{lo}-{2 * lo - 1}"
+            tiles_hist[key] = tiles_hist.get(key, 0) + 1
         case = {"dtype": tag, "ms": round(t * 1e3, 3), "gflops": round(2.0 * nnz / t / 1e9, 1),
                 "gbs": round(B / t / 1e9, 1), "frac_of_peak": round(B / t / 1e9 / HBM_PEAK_GBS, 4),
                 "bytes": B, "tiles": info["ntiles"], "persistent_waves": info["waves"], "launch": "hipGraph",
                 "waves_candidates_ms": {str(k): round(v * 1e3, 3) for k, v in wave_times[srt].items()},
-                "tile_values_histogram": {str(k): v for k, v in sorted(tiles_hist.items())},
+                "tile_values_histogram": dict(sorted(tiles_hist.items(), key=lambda kv: int(kv[0].split("-")[0]))),
                 "row_sort": srt, "ms_by_row_sort": {str(k): round(v * 1e3, 3) for k, v in times.items()}}
         out["cases"].append(case)
         log(f"[bench] batch {tag}: {t * 1e3:.2f} ms {case['gflops']} GFLOP/s {case['gbs']} GB/s "
