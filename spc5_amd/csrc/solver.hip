@@ -40,12 +40,17 @@ using namespace spc5;
 namespace spc5 {
 namespace {
 
+// Workgroups of the first stage of the p.q reduction (see k_reduce_stage).
+constexpr int kSumGroups = 64;
+
 // Device scalars of CG.  Iteration i reads r.r from rr[i & 1] and writes
-// the new r.r to rr[(i + 1) & 1], so no kernel has to rotate them.
+// the new r.r to rr[(i + 1) & 1], so no kernel has to rotate them.  pq_stage
+// holds the first-stage sums of the product's p.q partials.
 struct CgScalars {
     double rr[2];
     double pq;
     double pad;
+    double pq_stage[kSumGroups];
 };
 
 constexpr int kVecThreads = 256;
@@ -159,13 +164,46 @@ __global__ __launch_bounds__(kVecThreads) void k_add_scaled_f32(double* __restri
         x[i] += s * (double)d[i];
 }
 
-// K2: r -= alpha q; partials of r.r (par: parity of the iteration)
+// First stage of the p.q reduction, spread over kSumGroups workgroups (one
+// workgroup summing every partial spends its time issuing loads through one
+// CU): workgroup w sums the contiguous range [w c, (w+1) c) of the n
+// partials, c = ceil(n / kSumGroups), in a fixed order (thread t adds
+// elements t, t + kVecThreads, ... of the range, 4 loads in flight; then the
+// wave and workgroup sums) into out[w].  k_cg_r adds the kSumGroups sums.
+__global__ __launch_bounds__(kVecThreads) void k_reduce_stage(const double* __restrict__ part, int64_t n,
+                                                              double* __restrict__ out) {
+    __shared__ double s_part[kVecThreads / 64];
+    const int64_t c = (n + kSumGroups - 1) / kSumGroups;
+    const int64_t b = (int64_t)blockIdx.x * c;
+    const int64_t e = b + c < n ? b + c : n;
+    double acc = 0.0;
+    for (int64_t i0 = b; i0 < e; i0 += 4 * kVecThreads) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t i = i0 + threadIdx.x + (int64_t)u * kVecThreads;
+            v[u] = i < e ? part[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += v[u];
+    }
+    acc = block_sum(acc, s_part);
+    if (threadIdx.x == 0) out[blockIdx.x] = acc;
+}
+
+// K2: p.q = sum of the kSumGroups first-stage sums (workgroup 0 stores it in
+// sc->pq for K3); r -= alpha q; partials of r.r (par: parity of the
+// iteration)
 template <class X>
 __global__ __launch_bounds__(kVecThreads) void k_cg_r(X* __restrict__ r, const X* __restrict__ q, int64_t n,
-                                                      const CgScalars* __restrict__ sc, int par,
-                                                      double* __restrict__ part) {
+                                                      CgScalars* __restrict__ sc, int par, double* __restrict__ part) {
+    static_assert(kSumGroups == 64, "one first-stage sum per lane");
     __shared__ double s_part[kVecThreads / 64];
-    const X alpha = (X)(sc->rr[par] / sc->pq);
+    // every wave adds the same 64 sums by the same xor butterfly: one value
+    // in every lane of every workgroup
+    const double pq = wave_sum_f64(sc->pq_stage[threadIdx.x & 63]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->pq = pq;
+    const X alpha = (X)(sc->rr[par] / pq);
     double p = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads) {
         const X ri = r[i] - alpha * q[i];
@@ -362,7 +400,8 @@ int cg_run(const spc5_hip_mat& A, X* x, X* r, X* p, X* q, CgScalars* sc, double*
     auto iteration = [&](hipStream_t t, int par) -> int {
         int e;
         if ((e = launch_spmv(A, p, q, t, ek1))) return e;
-        hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kSumThreads), 0, t, part, A.plan.ntiles, &sc->pq);
+        hipLaunchKernelGGL(k_reduce_stage, dim3(kSumGroups), dim3(kVecThreads), 0, t, part, A.plan.ntiles,
+                           sc->pq_stage);
         hipLaunchKernelGGL(k_cg_r<X>, dim3(g), dim3(kVecThreads), 0, t, r, q, n, sc, par, part);
         hipLaunchKernelGGL(k_cg_xp<X>, dim3(g), dim3(kVecThreads), 0, t, x, p, r, n, sc, par, part, g);
         SPC5_HIP_TRY(hipGetLastError());
@@ -547,7 +586,8 @@ int mixed_cg_impl(const spc5_hip_mat& A64, const spc5_hip_mat& Alo, const double
     auto iteration = [&](hipStream_t t, int par, bool last) -> int {
         int e;
         if ((e = spmv_lo(t))) return e;
-        hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kSumThreads), 0, t, part, Alo.plan.ntiles, &sc->pq);
+        hipLaunchKernelGGL(k_reduce_stage, dim3(kSumGroups), dim3(kVecThreads), 0, t, part, Alo.plan.ntiles,
+                           sc->pq_stage);
         hipLaunchKernelGGL(k_cg_r<XV>, dim3(g), dim3(kVecThreads), 0, t, r, q, n, sc, par, part);
         if (!last)
             hipLaunchKernelGGL(k_cg_xp<XV>, dim3(g), dim3(kVecThreads), 0, t, d, p, r, n, sc, par, part, g);

@@ -217,6 +217,33 @@ def test_mixed_cg_reliable_updates(kind, r, vectors, tail):
     assert np.linalg.norm(xh - xs) / np.linalg.norm(xs) <= 1e-7
 
 
+def test_mixed_cg_tail_with_another_r_and_many_tiles():
+    """The double tail runs A64's product, whose tile partials go to the
+    buffer the float product uses: with A64 at another r than the float
+    matrix, more tiles than the float matrix and more than 2,048 tiles (the
+    vector kernels' partial count), the buffer must hold A64's longer list."""
+    import scipy.sparse as sp
+
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import lowp, mixed_cg
+    rp, col, val = stencil27_small(64)
+    n = len(rp) - 1
+    A64 = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val), 1)
+    A64.tile_values = 1024
+    base = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val), 4)
+    base.tile_values = 2048
+    Alo = lowp(base, "f32")
+    nt64, ntlo = A64.info()["ntiles"], Alo.info()["ntiles"]
+    assert nt64 > ntlo > 2048, (nt64, ntlo)
+    b = np.random.default_rng(8).uniform(-1, 1, n)
+    x, st = mixed_cg(A64, Alo, dev(b), rtol=1e-10, tail_rtol=1e-6)
+    assert st["tail_iterations"] > 0
+    A = sp.csr_matrix((val, col, rp), shape=(n, n))
+    xh = x.cpu().numpy()
+    relres = np.linalg.norm(b - A @ xh) / np.linalg.norm(b)
+    assert relres <= 1e-10, (relres, st)
+
+
 def test_cg_graph_replay_matches_direct_launches(monkeypatch):
     """The CG chunk recorded into a hipGraph and replayed gives the same
     iterates, bit for bit, as launching every kernel directly (double CG and
