@@ -76,6 +76,25 @@ def time_spmv(m, x, y, steps, warmup):
     return e0.elapsed_time(e1) / steps, wall
 
 
+def step_times(m, x, y, steps):
+    """Per-launch times (ms) of `steps` back-to-back products: an event
+    before every launch and one after the last, on the launch stream.
+    Returns (median, min)."""
+    import statistics
+
+    import torch
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
+    for i in range(steps):
+        ev[i].record(stream)
+        m.spmv(x, y)
+    ev[steps].record(stream)
+    torch.cuda.synchronize()
+    t = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
+    return statistics.median(t), min(t)
+
+
 def copy_bandwidth():
     """Device-to-device copy of 2 GiB (read + write counted)."""
     import torch
@@ -592,6 +611,7 @@ def main():
         # the tile size is the autotuner's per-matrix choice (timed candidates)
         tile, tile_times = autotune.tune_tiles(m) if not args.no_tune_tiles else (0, {})
         ms, wall = time_spmv(m, x, y, args.steps, args.warmup)
+        ms_med, ms_min = step_times(m, x, y, args.steps)
         B = m.algorithmic_bytes()
         gbs = B / (ms * 1e-3) / 1e9
         gflops = 2.0 * nnz / (ms * 1e-3) / 1e9
@@ -599,13 +619,15 @@ def main():
         formats[m.format_name] = {"r": r, "nblocks": m.nblocks, "ntiles": info["ntiles"],
                                   "tile_values": m.tile_target(), "tile_chosen_by": "autotune.tune_tiles",
                                   "tile_candidates_ms": {str(k): round(v * 1e3, 5) for k, v in tile_times.items()},
-                                  "bytes_per_spmv": B, "ms": round(ms, 5), "gflops": round(gflops, 2),
+                                  "bytes_per_spmv": B, "ms": round(ms, 5), "ms_median": round(ms_med, 5),
+                                  "ms_min": round(ms_min, 5), "gflops": round(gflops, 2),
                                   "gbs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
                                   "convert_s": round(conv_s, 3), "convert_ms": round(conv_ms, 3),
                                   "convert_bytes": conv_bytes,
                                   "convert_gbs": round(conv_bytes / (conv_ms * 1e-3) / 1e9, 1),
                                   "timed_wall_s": round(wall, 3)}
-        log(f"[bench] beta({r},8): {ms:.4f} ms  {gflops:.1f} GFLOP/s  {gbs:.0f} GB/s  "
+        log(f"[bench] beta({r},8): {ms:.4f} ms (median {ms_med:.4f}, min {ms_min:.4f})  {gflops:.1f} GFLOP/s  "
+            f"{gbs:.0f} GB/s  "
             f"({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)  nb={m.nblocks} tiles={info['ntiles']} "
             f"convert {conv_ms:.2f} ms (first {1e3 * conv_s:.2f}) "
             f"(tile size {m.tile_target()} values; candidates "
@@ -633,10 +655,13 @@ def main():
         for r in [int(s) for s in args.formats.split(",")]:
             m = DeviceMatrix.from_csr(d_rp, d_col, d_val32, r)
             ms, _ = time_spmv(m, x32, y32, args.steps, args.warmup)
+            ms_med, ms_min = step_times(m, x32, y32, args.steps)
             B = m.algorithmic_bytes()
             gbs = B / (ms * 1e-3) / 1e9
             formats_f32[m.format_name] = {"r": r, "vs": m.vs, "nblocks": m.nblocks, "bytes_per_spmv": B,
-                                          "ms": round(ms, 5), "gflops": round(2.0 * nnz / (ms * 1e-3) / 1e9, 2),
+                                          "ms": round(ms, 5), "ms_median": round(ms_med, 5),
+                                          "ms_min": round(ms_min, 5),
+                                          "gflops": round(2.0 * nnz / (ms * 1e-3) / 1e9, 2),
                                           "gbs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
             log(f"[bench] FP32 beta({r},{m.vs}): {ms:.4f} ms  {gbs:.0f} GB/s  "
                 f"({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)")
