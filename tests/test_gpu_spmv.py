@@ -78,6 +78,54 @@ def test_gpu_converter_matches_reference_bytes(case, tag, r):
     assert block_count(dev(rp), dev(col), r, O.VS_OF[np.dtype(DT[tag])]) == int(g[f"bc_{tag}_r{r}"])
 
 
+def _stage_overflow_csr(kind, dt, seed):
+    """CSR matrices of 40 groups of 64 rows (the converter's groups) in
+    which one group outgrows the LDS stage the launch sizes from the mean
+    group: "values" gives group 1 far more values than the others; "headers"
+    gives group 1 as many values as the others but one block per value
+    (scattered columns), so only its header bytes outgrow the stage."""
+    rng = np.random.default_rng(seed)
+    n, ncols = 64 * 40, 20000
+    sets = []
+    for i in range(n):
+        g = i // 64
+        base = (i * 7) % (ncols - 400)
+        if kind == "values" and g == 1:
+            sets.append(np.arange(base, base + 300, dtype=np.int64))
+        elif kind == "headers" and g == 1:
+            sets.append(np.arange(16, dtype=np.int64) * 33 + (i % 5))  # 16 scattered columns
+        elif kind == "headers":
+            sets.append(np.arange(base, base + 16, dtype=np.int64))  # two full runs of 8
+        else:
+            sets.append(np.sort(rng.choice(np.arange(base, base + 40), 4, replace=False)))
+    rp = np.zeros(n + 1, np.int32)
+    for i, c in enumerate(sets):
+        rp[i + 1] = rp[i] + len(c)
+    col = np.concatenate(sets).astype(np.int32)
+    v = rng.uniform(-1, 1, len(col)).astype(dt)
+    return rp, col, v, ncols
+
+
+@pytest.mark.parametrize("kind", ["values", "headers"])
+@pytest.mark.parametrize("tag", ["f64", "f32"])
+@pytest.mark.parametrize("r", RS)
+def test_gpu_converter_stage_overflow_bytes(kind, tag, r):
+    """Groups too large for the converter's LDS stage (the global-memory
+    path) next to staged ones: bytes equal to the oracle's converter, and the
+    count pass equal to its block count."""
+    from spc5_amd.device import block_count, convert_csr
+    rp, col, v, ncols = _stage_overflow_csr(kind, DT[tag], r)
+    n = len(rp) - 1
+    nbr = (n + r - 1) // r
+    bptr, hdr, vout, nb = O.convert(r, rp, col, v)
+    rs_g, hdr_g, vout_g, nb_g = convert_csr(dev(rp), dev(col), dev(v), r)
+    assert nb_g == nb
+    assert np.array_equal(host(rs_g), np.asarray(bptr[: nbr + 1], np.int32))
+    assert np.array_equal(host(hdr_g), np.asarray(hdr, np.uint8))
+    assert np.array_equal(host(vout_g).view(np.uint8), np.ascontiguousarray(vout).view(np.uint8))
+    assert block_count(dev(rp), dev(col), r, O.VS_OF[np.dtype(DT[tag])]) == nb
+
+
 @pytest.mark.parametrize("case", golden_cases())
 @pytest.mark.parametrize("tag", ["f64", "f32"])
 @pytest.mark.parametrize("r", RS)
