@@ -4,7 +4,8 @@
 // The reference walks each block-row with one cursor per row: a block starts
 // at the smallest unconsumed column of the block-row and takes, row by row,
 // every entry with column < start+VS.  Block-rows are independent, so the
-// GPU runs one lane per block-row in two passes:
+// GPU gives every 64 rows to one wave, one lane per row, the r lanes of a
+// block-row walking together (see "Groups" below), in two passes:
 //   count : blocks per block-row -> exclusive scan -> block-row pointers
 //           (the reference's rowsSize after conversion) and the block count;
 //   fill  : the same walk again, writing each block's packed header at
@@ -333,7 +334,8 @@ __global__ __launch_bounds__(64) void k_convert_fill(int64_t nrows, int64_t nbr,
     const int b_lo = bptr[br < nbr ? br : nbr];
     const int b_hi = bptr[br + 1 < nbr ? br + 1 : nbr];
     const int64_t blk0 = __shfl(b_lo, 0, 64);
-    const int64_t blkE = (int64_t)__shfl(b_hi, 63, 64) < nblocks ? (int64_t)__shfl(b_hi, 63, 64) : nblocks;
+    const int64_t b_last = __shfl(b_hi, 63, 64);  // pointer past the group's last block-row
+    const int64_t blkE = b_last < nblocks ? b_last : nblocks;
     int nblk = 0, bo = 0;  // blocks of the lane's block-row, its first block's index in the group
     if (br < nbr) {
         const int64_t b_end = b_hi < nblocks ? b_hi : nblocks;
