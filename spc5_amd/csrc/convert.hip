@@ -128,25 +128,26 @@ __device__ __forceinline__ int group_excl_scan(int v, int t) {
 // block, first the value position (block order) of the lane's first value in
 // the block, n its count and mask its VS-bit row mask; the lane's values were
 // at source positions [src0, src0 + n), whose columns the walk never reads
-// again.  Returns whether a lane saw columns that break the format's
-// precondition (non-negative, strictly increasing).
+// again.  With CHECK, returns whether a lane saw columns that break the
+// format's precondition (non-negative, strictly increasing); without, the
+// caller has checked them and it returns false.
 //
 // `col` is indexed by global value position minus `base` (0 for the global
 // array, the group's first value for the stage).  Per block a lane reads the
 // window of its next VS + 1 columns at once: its values in the block are the
 // window's leading run below start + VS (at most VS, the columns being
 // strictly increasing), the entry after them is the next head, and adjacent
-// window entries check the precondition.  PADDED: `col` may be read up to
+// window entries check the precondition (CHECK).  PADDED: `col` may be read up to
 // VS + 1 entries past the group's last value (the stage's pad), so the window
 // is read without clamping; otherwise each index is clamped to the row.
-template <int R, int VS, bool PADDED, class C, class F>
+template <int R, int VS, bool PADDED, bool CHECK, class C, class F>
 __device__ __forceinline__ bool walk_group(int rb, int re, C col, int base, F on_block) {
     const int lane = threadIdx.x;
     const int t = lane % R;
     const int end = re;
     int cur = rb;
     int head = cur < end ? col[cur - base] : INT_MAX;  // INT_MAX once the row is consumed
-    bool bad = head < 0;
+    bool bad = CHECK && head < 0;
     // the block-row's first value: the row pointer of its row 0
     int vbase = __shfl(rb, lane - t, 64);
     int k = 0;
@@ -174,8 +175,10 @@ __device__ __forceinline__ bool walk_group(int rb, int re, C col, int base, F on
                 mask |= alive ? 1u << (d & 31u) : 0u;
                 n += alive ? 1 : 0;
             }
+            if constexpr (CHECK) {
 #pragma unroll
-            for (int j = 1; j <= VS; ++j) bad |= cur + j < end && w[j] <= w[j - 1];
+                for (int j = 1; j <= VS; ++j) bad |= cur + j < end && w[j] <= w[j - 1];
+            }
             int next = w[0];
 #pragma unroll
             for (int j = 1; j <= VS; ++j) next = n == j ? w[j] : next;
@@ -279,9 +282,36 @@ __device__ __forceinline__ GroupRows group_rows(const int* __restrict__ rowptr, 
     return g;
 }
 
+// Row-start bitmap words after a stage of sv values (bit i: value i starts
+// a row; one spare word).
+__host__ __device__ constexpr int row_start_words(int sv) { return sv / 32 + 2; }
+
+// The column precondition over a group's staged columns (non-negative,
+// strictly increasing inside each row), one coalesced pass: value i and its
+// successor are compared unless the successor starts a row, as the lanes mark
+// their rows' first values in the bitmap `rs`.  True when broken.
+__device__ __forceinline__ bool staged_columns_bad(const int* scol, unsigned* rs, int nv, int rb, int re, int v0,
+                                                   int lane) {
+    const int nw = nv / 32 + 2;
+    for (int i = lane; i < nw; i += 64) rs[i] = 0u;
+    wave_sync();
+    if (rb < re) atomicOr(&rs[(rb - v0) >> 5], 1u << ((rb - v0) & 31));
+    wave_sync();
+    bool bad = false;
+#pragma unroll 4
+    for (int i = lane; i < nv; i += 64) {
+        const int c0 = scol[i];
+        const int c1 = scol[i + 1];  // the stage's pad when i + 1 == nv
+        const bool next_starts_row = i + 1 >= nv || ((rs[(i + 1) >> 5] >> ((i + 1) & 31)) & 1u);
+        bad |= c0 < 0 || (!next_starts_row && c1 <= c0);
+    }
+    return bad;
+}
+
 // Blocks per block-row (the count pass).  A violation of the column
 // precondition sets bit 0 of *err; the fill pass is then never run.  Dynamic
-// LDS: sv staged column indices and kStagePad words of pad.
+// LDS: sv staged column indices, kStagePad words of pad, then
+// row_start_words(sv) words of row-start bitmap.
 template <int R, int VS>
 __global__ __launch_bounds__(64) void k_convert_count(int64_t nrows, int64_t nbr, const int* __restrict__ rowptr,
                                                       const int* __restrict__ col, int* __restrict__ cnt_out,
@@ -289,21 +319,23 @@ __global__ __launch_bounds__(64) void k_convert_count(int64_t nrows, int64_t nbr
     constexpr int BR = group_blockrows<R>();
     extern __shared__ __align__(16) unsigned char smem[];
     int* scol = reinterpret_cast<int*>(smem);
+    unsigned* rstart = reinterpret_cast<unsigned*>(smem + 4 * (size_t)(sv + kStagePad));
     const int lane = threadIdx.x;
     const int64_t br0 = (int64_t)blockIdx.x * BR;
     const GroupRows g = group_rows(rowptr, br0 * R, nrows, lane);
     int nb = 0;
     auto on_block = [&](int, int, int, int, int, unsigned, int) { ++nb; };
-    bool bad;
+    bool bad = false;
     if (g.nv <= sv) {
         if (g.nv > 0) {
             int c[kPrefetch];
             prefetch_words(col + g.v0, g.nv, lane, c);
             stage_columns(col + g.v0, scol, g.nv, lane, c);
+            bad = staged_columns_bad(scol, rstart, g.nv, g.rb, g.re, g.v0, lane);
         }
-        bad = walk_group<R, VS, true>(g.rb, g.re, scol, g.v0, on_block);
+        walk_group<R, VS, true, false>(g.rb, g.re, scol, g.v0, on_block);
     } else {
-        bad = walk_group<R, VS, false>(g.rb, g.re, col, 0, on_block);
+        bad = walk_group<R, VS, false, true>(g.rb, g.re, col, 0, on_block);
     }
     const int64_t br = br0 + lane / R;
     if (lane % R == 0 && br < nbr) cnt_out[br] = nb;
@@ -348,8 +380,8 @@ __global__ __launch_bounds__(64) void k_convert_fill(int64_t nrows, int64_t nbr,
     const bool staged = nv <= sv && (g1 - g0) + align <= shb;
     if (!staged) {
         // too large for the stage: global columns, direct writes
-        walk_group<R, VS, false>(g.rb, g.re, col, 0, [&](int k, int start, int tt, int first, int n, unsigned mask,
-                                                          int src0) {
+        walk_group<R, VS, false, false>(g.rb, g.re, col, 0, [&](int k, int start, int tt, int first, int n,
+                                                                 unsigned mask, int src0) {
             if (k >= nblk) return;  // inconsistent pointers: never write past the block-row
             unsigned char* h = hdr + (blk0 + bo + k) * H;
             if (tt == 0) __builtin_memcpy(h, &start, 4);
@@ -365,8 +397,8 @@ __global__ __launch_bounds__(64) void k_convert_fill(int64_t nrows, int64_t nbr,
     prefetch_words(col + v0, nv, lane, c);
     prefetch_words(vin + v0, nv, lane, v);
     stage_columns(col + v0, scol, nv, lane, c);
-    walk_group<R, VS, true>(g.rb, g.re, scol, v0, [&](int k, int start, int tt, int first, int n, unsigned mask,
-                                                      int src0) {
+    walk_group<R, VS, true, false>(g.rb, g.re, scol, v0, [&](int k, int start, int tt, int first, int n,
+                                                             unsigned mask, int src0) {
         if (k >= nblk) return;
         unsigned char* h = shdr + align + (bo + k) * H;
         if (tt == 0) __builtin_memcpy(h, &start, 4);
@@ -424,8 +456,8 @@ void launch_count(int64_t nrows, int64_t nbr, const int* rowptr, const int* col,
                   hipStream_t s) {
     const int64_t groups = (nbr + group_blockrows<R>() - 1) / group_blockrows<R>();
     const int sv = stage_values(nnz, groups);
-    hipLaunchKernelGGL((k_convert_count<R, VS>), dim3((unsigned)groups), dim3(64), (size_t)(sv + kStagePad) * 4, s,
-                       nrows, nbr,
+    const size_t shm = (size_t)(sv + kStagePad + row_start_words(sv)) * 4;
+    hipLaunchKernelGGL((k_convert_count<R, VS>), dim3((unsigned)groups), dim3(64), shm, s, nrows, nbr,
                        rowptr, col, out, err, sv);
 }
 

@@ -126,6 +126,62 @@ def test_gpu_converter_stage_overflow_bytes(kind, tag, r):
     assert block_count(dev(rp), dev(col), r, O.VS_OF[np.dtype(DT[tag])]) == nb
 
 
+def _rows_csr(rows):
+    rp = np.zeros(len(rows) + 1, np.int32)
+    for i, c in enumerate(rows):
+        rp[i + 1] = rp[i] + len(c)
+    col = np.concatenate([np.asarray(c, np.int64) for c in rows]).astype(np.int32)
+    return rp, col
+
+
+@pytest.mark.parametrize("r", RS)
+@pytest.mark.parametrize("big", [False, True])
+def test_gpu_converter_column_precondition(r, big):
+    """Columns must be non-negative and strictly increasing inside each row:
+    a decrease, a repeat or a negative column anywhere is refused by the
+    count pass; a drop across a row boundary (empty rows between included)
+    is not.  `big` puts the rows in a 64-row group too large for the LDS
+    stage (the global-memory walk) beside small groups."""
+    from spc5_amd import Spc5Error
+    from spc5_amd.device import block_count, convert_csr
+    rng = np.random.default_rng(r)
+    base = []
+    for i in range(64 * 6):
+        if i % 7 == 3:
+            base.append([])
+        elif big and 64 <= i < 128:
+            base.append(list(range(i, i + 400)))
+        else:
+            base.append(sorted(rng.choice(np.arange(50, 2000), 12, replace=False).tolist()))
+    row = 70  # inside the big group when big
+    good_rp, good_col = _rows_csr(base)
+    v = rng.uniform(-1, 1, len(good_col))
+    assert block_count(dev(good_rp), dev(good_col), r) > 0
+    convert_csr(dev(good_rp), dev(good_col), dev(v), r)
+    for fault in ("swap", "repeat", "negative_first", "negative_later"):
+        rows = [list(c) for c in base]
+        c = rows[row]
+        if fault == "swap":
+            c[4], c[5] = c[5], c[4]
+        elif fault == "repeat":
+            c[6] = c[5]
+        elif fault == "negative_first":
+            c[0] = -3
+        else:
+            c[7] = -1
+        rp, col = _rows_csr(rows)
+        with pytest.raises(Spc5Error, match="increasing"):
+            block_count(dev(rp), dev(col), r)
+        with pytest.raises(Spc5Error, match="increasing"):
+            convert_csr(dev(rp), dev(col), dev(v[: len(col)]), r)
+    # the last column of a row above the first of the next: accepted
+    rows = [list(c) for c in base]
+    rows[row] = [1990, 1995]
+    rows[row + 1] = [0, 1]
+    rp, col = _rows_csr(rows)
+    assert block_count(dev(rp), dev(col), r) > 0
+
+
 @pytest.mark.parametrize("case", golden_cases())
 @pytest.mark.parametrize("tag", ["f64", "f32"])
 @pytest.mark.parametrize("r", RS)
