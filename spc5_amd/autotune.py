@@ -254,7 +254,7 @@ def _time_fn(fn, reps=10, warm=2):
     return e0.elapsed_time(e1) / reps * 1e-3
 
 
-def tune_tiles(m: DeviceMatrix, candidates=(0, 1024, 2048, 4096, 8192), reps=10):
+def tune_tiles(m: DeviceMatrix, candidates=(0, 1024, 1536, 2048, 3072, 4096, 8192), reps=20):
     """Fastest tile size (values per wave's tile, SPC5_HIP_OPT_TILE_VALUES;
     0 is the default rule) for the matrix's product; leaves the matrix set
     to it.  Returns (best, {candidate: seconds})."""
