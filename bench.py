@@ -419,17 +419,20 @@ def spmm_benchmark(n=160, ks=(1, 4, 16, 64), steps=10, warmup=2, rs=(2, 4, 8)):
             vt = next(iter(mats.values())).vec_dtype
             X = torch.empty((N, k), dtype=vt, device="cuda").uniform_(-1, 1, generator=g)
             Y = torch.zeros((N, k), dtype=vt, device="cuda")
-            (r, kt, cols), cand = autotune.tune_spmm(mats, k)
+            (r, kt, cols, x16), cand = autotune.tune_spmm(mats, k)
             A = mats[r]
-            A.spmm_columns = cols
+            A.spmm_columns, A.spmm_x16 = cols, x16
             t = _events_time(lambda: A.spmm(X, Y, kt=kt), steps, warmup)
             a_bytes = A.algorithmic_bytes() - (A.ncols + 2 * A.nrows) * A.vec_dtype.itemsize
             B = a_bytes + k * (A.ncols + 2 * A.nrows) * A.vec_dtype.itemsize
-            case = {"dtype": tag, "k": k, "r": r, "kt": kt, "records": "per block column" if cols else "per value",
+            kinds = {(False, False): "values", (True, False): "columns", (True, True): "columns x16"}
+            case = {"dtype": tag, "k": k, "r": r, "kt": kt,
+                    "records": ("per block column, 16-byte X requests" if x16 else "per block column") if cols
+                    else "per value",
                     "ms": round(t * 1e3, 4), "gflops": round(2.0 * nnz * k / t / 1e9, 1),
                     "gbs": round(B / t / 1e9, 1), "frac_of_peak": round(B / t / 1e9 / HBM_PEAK_GBS, 4), "bytes": B,
-                    "candidates_ms": {f"r={cr} kt={ckt} {'columns' if cc else 'values'}": round(v * 1e3, 4)
-                                      for (cr, ckt, cc), v in sorted(cand.items())}}
+                    "candidates_ms": {f"r={cr} kt={ckt} {kinds[(cc, cx)]}": round(v * 1e3, 4)
+                                      for (cr, ckt, cc, cx), v in sorted(cand.items())}}
             out["cases"].append(case)
             log(f"[bench] SpMM {tag} k={k} beta({r},8) kt={kt} {case['records']}: {t * 1e3:.3f} ms "
                 f"{case['gflops']} GFLOP/s {case['gbs']} GB/s")

@@ -158,6 +158,15 @@ int spc5_hip_mat_load(const char* path, spc5_hip_mat_t* out);
  * batch created afterwards uses the new tiles.  No counterpart in the
  * reference (the work split of the GPU kernel). */
 #define SPC5_HIP_OPT_TILE_VALUES 3
+/* SPMM_X16 (0 or 1; default 1 for double vectors, 0 for float): with
+ * SPMM_COLUMNS, each lane requests 16 bytes of an X row (two doubles or four
+ * floats, adjacent columns) instead of one element, where X allows it
+ * (16-byte aligned X, a leading dimension, k and column chunk that are
+ * multiples of the element count); otherwise one element per lane.  Fewer
+ * lanes then share a record, which halves (double) or quarters (float) the
+ * address work per column.  Results agree with one element per lane to
+ * rounding.  No counterpart in the reference. */
+#define SPC5_HIP_OPT_SPMM_X16 4
 int spc5_hip_mat_set_option(spc5_hip_mat_t h, int option, int64_t value);
 
 /* y += A x on device pointers (x and y in the matrix's vector type).

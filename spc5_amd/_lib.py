@@ -19,6 +19,7 @@ SPC5_HIP_BF16 = 2
 SPC5_HIP_OPT_ROW_SORT = 1
 SPC5_HIP_OPT_SPMM_COLUMNS = 2
 SPC5_HIP_OPT_TILE_VALUES = 3
+SPC5_HIP_OPT_SPMM_X16 = 4
 
 _P = C.c_void_p
 _I = C.c_int

@@ -222,20 +222,24 @@ def tune_spmm(mats, k: int, kt_candidates=None, reps=5):
     """Fastest schedule for Y += A X with k right-hand sides over candidate
     block shapes of one matrix: `mats` maps r -> DeviceMatrix (the same
     matrix converted with that r).  Tries every r, column chunk kt and
-    record kind (one record per stored value, or per set column of a block
-    with r > 1, SPC5_HIP_OPT_SPMM_COLUMNS) and leaves each matrix set to its
-    best record kind.  Returns ((r, kt, columns), {(r, kt, columns): seconds})."""
+    record kind: one record per stored value, or (r > 1) one per set column
+    of a block (SPC5_HIP_OPT_SPMM_COLUMNS), whose X requests are one element
+    or 16 bytes per lane (SPC5_HIP_OPT_SPMM_X16).  Leaves each matrix set to
+    its best record kind.  Returns ((r, kt, columns, x16),
+    {(r, kt, columns, x16): seconds}); x16 is False for per-value records."""
     times = {}
     for r, m in mats.items():
-        best_cols, best_t = False, None
-        for cols in ((False, True) if r > 1 else (False,)):
+        best_kind, best_t = (False, False), None
+        kinds = ((False, False), (True, False), (True, True)) if r > 1 else ((False, False),)
+        for cols, x16 in kinds:
             m.spmm_columns = cols
+            m.spmm_x16 = x16
             _, res = tune_spmm_kt(m, k, kt_candidates, reps)
             for kt, t in res.items():
-                times[(r, kt, cols)] = t
+                times[(r, kt, cols, x16)] = t
                 if best_t is None or t < best_t:
-                    best_t, best_cols = t, cols
-        m.spmm_columns = best_cols
+                    best_t, best_kind = t, (cols, x16)
+        m.spmm_columns, m.spmm_x16 = best_kind
     best = min(times, key=times.get)
     return best, times
 

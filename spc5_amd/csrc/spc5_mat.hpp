@@ -85,6 +85,7 @@ struct spc5_hip_mat {
     const void* values = nullptr;         // device, [nnz]
     int row_sort = 0;                     // SPC5_HIP_OPT_ROW_SORT
     int spmm_cols = 0;                    // SPC5_HIP_OPT_SPMM_COLUMNS
+    int spmm_x16 = -1;                    // SPC5_HIP_OPT_SPMM_X16; -1: 1 for double vectors, else 0
     spc5::Plan plan;
     std::vector<std::shared_ptr<spc5::DeviceBuffer>> owned;  // buffers the handle allocated itself
 };

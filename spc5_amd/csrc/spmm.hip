@@ -82,6 +82,7 @@ static_assert(SPC5_SPMM_CAP64_NARROW >= 128 && SPC5_SPMM_CAP64_NARROW % 64 == 0,
 constexpr int kSpmmAccBytes = SPC5_SPMM_ACC_BYTES;
 constexpr int kSpmmMaxKt = 64;       // columns per pass (one lane each)
 
+
 // One value of a batch: the value, where its X row is (a byte offset from
 // X when X spans less than 4 GiB, else the column) and its tile-local row.
 template <class ACC>
@@ -127,6 +128,29 @@ __device__ __forceinline__ double read_lane(double v, int l) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// The (tile, column chunk) pair of this wave.  The 1-D grid lists the pairs
+// chunk by chunk, tiles in order, padded to a multiple of 8.  Workgroups are
+// dealt to the 8 XCDs round-robin (group g runs on XCD g % 8), so groups are
+// renumbered so that each XCD walks one contiguous eighth of that list: the
+// tiles an XCD holds at once are neighbours, and the X rows they gather (the
+// column footprint of one contiguous run of rows) stay in its own L2 instead
+// of every XCD's L2 holding the footprint of the whole front.  False for the
+// padding.
+__device__ __forceinline__ bool spmm_item(int64_t ntiles, int k, int kt, int64_t& t, int& c0) {
+    const unsigned g = blockIdx.x;
+#if SPC5_XCD_REMAP
+    const unsigned per_xcd = gridDim.x >> 3;
+    const unsigned gx = (g & 7u) * per_xcd + (g >> 3);
+#else
+    const unsigned gx = g;
+#endif
+    const unsigned nt = (unsigned)ntiles;  // ntiles * chunks < 2^32 (checked at launch)
+    const unsigned ch = gx / nt;
+    t = (int64_t)(gx - ch * nt);
+    c0 = (int)ch * kt;
+    return (int64_t)ch * kt < k;  // false for the padding after the last chunk
+}
+
 // WIDE: X spans 4 GiB or more (or rows of 16 MiB or more), so records hold
 // columns and X is addressed with 64-bit arithmetic; otherwise they hold
 // 32-bit byte offsets of the X rows, computed with 24-bit multiplies.
@@ -148,9 +172,9 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
     ACC* acc = reinterpret_cast<ACC*>(rec + CAP);                       // [maxrows * kt]
     int* rs = reinterpret_cast<int*>(acc + (size_t)maxrows * kt);       // [maxrows / R + 1]
 
-    const int64_t t = blockIdx.x;
-    if (t >= ntiles) return;
-    const int c0 = blockIdx.y * kt;
+    int64_t t;
+    int c0;
+    if (!spmm_item(ntiles, k, kt, t, c0)) return;
     const int ktc = (k - c0) < kt ? (k - c0) : kt;
     const int lane = lane_id();
     const int grp = lane / KC;
@@ -327,11 +351,33 @@ constexpr int spmm_col_cap_rec() { return R == 2 ? 128 : (R == 4 ? 96 : 64); }
 template <int R>
 constexpr int spmm_col_cap_val() { return R == 2 ? 256 : (R == 4 ? 384 : 512); }
 
-// With 64-column chunks one wave walks the whole record sequence of its
-// tile in block-row order, so it writes each finished block-row of Y
-// directly (no LDS row x column sums, tiles of up to kMaxRows rows).
-template <int KC>
-constexpr bool spmm_col_direct() { return KC == 64; }
+// With 64-column chunks of one column per lane, one wave walks the whole
+// record sequence of its tile in block-row order, so it writes each finished
+// block-row of Y directly (no LDS row x column sums, tiles of up to kMaxRows
+// rows).
+template <int KC, int CPL>
+constexpr bool spmm_col_direct() { return KC == 64 && CPL == 1; }
+
+// CPL adjacent X columns of one record: one X value, or (CPL = 16 /
+// sizeof(X)) one 16-byte load of two doubles or four floats; the caller has
+// checked that every X row start and every lane's first column are then
+// 16-byte aligned.
+template <int CPL, bool WIDE, class X, class ACC>
+__device__ __forceinline__ void spmm_xs(const X* __restrict__ xcol, const X* __restrict__ x, int col, int64_t ldx,
+                                        unsigned lane_off, ACC* out) {
+    if constexpr (CPL == 1) {
+        out[0] = (ACC)spmm_x<WIDE>(xcol, x, col, ldx, lane_off);
+    } else {
+        static_assert(CPL * sizeof(X) == 16, "one 16-byte load per lane");
+        const X* p = WIDE ? xcol + (int64_t)col * ldx
+                          : reinterpret_cast<const X*>(reinterpret_cast<const char*>(x) + ((unsigned)col + lane_off));
+        const uint4 w = *reinterpret_cast<const uint4*>(p);
+        X v[CPL];
+        __builtin_memcpy(v, &w, 16);
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) out[e] = (ACC)v[e];
+    }
+}
 
 template <class ACC, int R>
 size_t spmm_col_smem_bytes(int maxrows, int kt, bool direct) {
@@ -351,7 +397,9 @@ __device__ __forceinline__ unsigned block_columns(const unsigned long long* mw) 
     return cm;
 }
 
-template <class V, class X, class ACC, int R, int VS, int MB, int KC, bool WIDE>
+// KC: the chunk's columns rounded up to a power of two; CPL: adjacent
+// columns per lane (1, or 16 / sizeof(X)), so KC / CPL lanes share a record.
+template <class V, class X, class ACC, int R, int VS, int MB, int KC, bool WIDE, int CPL>
 __global__ __launch_bounds__(64) void k_spmm_cols(const int* __restrict__ bptr, const unsigned char* __restrict__ hdr,
                                                    const V* __restrict__ val, const X* __restrict__ x, int64_t ldx,
                                                    X* __restrict__ y, int64_t ldy, int k, int kt,
@@ -361,9 +409,15 @@ __global__ __launch_bounds__(64) void k_spmm_cols(const int* __restrict__ bptr, 
     constexpr int CAP = spmm_col_cap_val<R>();
     constexpr int K = CAP / 64;
     constexpr int NW = MaskWords<R * MB>::kWords;
-    constexpr int G = 64 / KC;           // record groups per wave
-    constexpr int U = R <= 2 ? 16 : 8;  // X rows requested together
-    constexpr bool DIRECT = spmm_col_direct<KC>();
+    constexpr int LPR = KC / CPL;        // lanes per record
+    constexpr int G = 64 / LPR;          // record groups per wave
+    // X rows requested together.  With several columns per lane and more
+    // than two groups per wave, half as many: k = 64 in double on the
+    // 27-point stencil at beta(4,8) took 3.16 ms with 32-column chunks and 4
+    // requests (3.53 ms with 8), and 2.95 ms with 64-column chunks (two
+    // groups) and 8 requests (3.12 ms with 4).
+    constexpr int U = (R <= 2 ? 16 : 8) / (CPL > 1 && G > 2 ? 2 : 1);
+    constexpr bool DIRECT = spmm_col_direct<KC, CPL>();
     using Rec = SpmmColRec<ACC, R>;
     static_assert(sizeof(Rec) % 16 == 0, "16-byte aligned records");
     extern __shared__ __align__(16) unsigned char smem[];
@@ -372,14 +426,14 @@ __global__ __launch_bounds__(64) void k_spmm_cols(const int* __restrict__ bptr, 
     int* rs = reinterpret_cast<int*>(acc + (DIRECT ? 0 : (size_t)maxrows * kt));      // [maxrows / R + 1]
     unsigned short* slot = reinterpret_cast<unsigned short*>(rs + (maxrows / R + 1));  // [CAP]
 
-    const int64_t t = blockIdx.x;
-    if (t >= ntiles) return;
-    const int c0 = blockIdx.y * kt;
+    int64_t t;
+    int c0;
+    if (!spmm_item(ntiles, k, kt, t, c0)) return;
     const int ktc = (k - c0) < kt ? (k - c0) : kt;
     const int lane = lane_id();
-    const int grp = lane / KC;
-    const int jl = lane % KC;
-    const bool colok = jl < ktc;
+    const int grp = lane / LPR;
+    const int jl = (lane % LPR) * CPL;  // the lane's first column in the chunk
+    const bool colok = jl < ktc;        // ktc is a multiple of CPL
     const X* __restrict__ xcol = x + c0 + (colok ? jl : 0);
     const unsigned lane_off = (unsigned)(c0 + (colok ? jl : 0)) * (unsigned)sizeof(X);
     const unsigned row_bytes = (unsigned)ldx * (unsigned)sizeof(X);
@@ -401,9 +455,11 @@ __global__ __launch_bounds__(64) void k_spmm_cols(const int* __restrict__ bptr, 
     // Running sums of the current block-row, one per row of it; with DIRECT
     // they persist across batches (the wave walks the tile's records in
     // block-row order) and finished block-rows go straight to Y.
-    ACC run[R];
+    ACC run[R][CPL];
 #pragma unroll
-    for (int tr = 0; tr < R; ++tr) run[tr] = (ACC)0;
+    for (int tr = 0; tr < R; ++tr)
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) run[tr][e] = (ACC)0;
     int pb = -1;    // block-row the running sums belong to
     int done = 0;   // DIRECT: block-rows [0, done) of the tile are written
     // DIRECT: Y of block-row pb, requested when the block-row opens so that
@@ -425,7 +481,7 @@ __global__ __launch_bounds__(64) void k_spmm_cols(const int* __restrict__ bptr, 
             const int rr = brow * R + tr;
             if (rr < nrow_t && colok) {
                 X* yp = &y[(row0 + rr) * ldy + c0 + jl];
-                const ACC s = have ? run[tr] : (ACC)0;
+                const ACC s = have ? run[tr][0] : (ACC)0;
                 const ACC yo = beta == (ACC)0 ? (ACC)0 : (have ? (ACC)yv[tr] : (ACC)*yp);
                 *yp = (X)(alpha * s + beta * yo);
             }
@@ -446,11 +502,15 @@ __global__ __launch_bounds__(64) void k_spmm_cols(const int* __restrict__ bptr, 
                 done = pb + 1;
             } else if (colok) {
 #pragma unroll
-                for (int tr = 0; tr < R; ++tr) atomicAdd(&acc[(pb * R + tr) * kt + jl], run[tr]);
+                for (int tr = 0; tr < R; ++tr)
+#pragma unroll
+                    for (int e = 0; e < CPL; ++e) atomicAdd(&acc[(pb * R + tr) * kt + jl + e], run[tr][e]);
             }
         }
 #pragma unroll
-        for (int tr = 0; tr < R; ++tr) run[tr] = (ACC)0;
+        for (int tr = 0; tr < R; ++tr)
+#pragma unroll
+            for (int e = 0; e < CPL; ++e) run[tr][e] = (ACC)0;
         pb = brow;
         request_y(brow);
     };
@@ -540,22 +600,27 @@ __global__ __launch_bounds__(64) void k_spmm_cols(const int* __restrict__ bptr, 
             }
             // all U X rows are requested before the first use waits
             __builtin_amdgcn_sched_barrier(0);
-            ACC xv[U];
+            ACC xv[U][CPL];
 #pragma unroll
-            for (int u = 0; u < U; ++u) xv[u] = (ACC)spmm_x<WIDE>(xcol, x, cu[u], ldx, lane_off);
+            for (int u = 0; u < U; ++u) spmm_xs<CPL, WIDE>(xcol, x, cu[u], ldx, lane_off, xv[u]);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (bu[u] != pb) flush(bu[u]);
 #pragma unroll
-                for (int tr = 0; tr < R; ++tr) run[tr] += rec[q + u].v[tr] * xv[u];
+                for (int tr = 0; tr < R; ++tr)
+#pragma unroll
+                    for (int e = 0; e < CPL; ++e) run[tr][e] += rec[q + u].v[tr] * xv[u][e];
             }
         }
         for (; q < qb; ++q) {
             const int2 hb = *reinterpret_cast<const int2*>(&rec[q]);
-            const ACC xv = (ACC)spmm_x<WIDE>(xcol, x, hb.x, ldx, lane_off);
+            ACC xv[CPL];
+            spmm_xs<CPL, WIDE>(xcol, x, hb.x, ldx, lane_off, xv);
             if (hb.y != pb) flush(hb.y);
 #pragma unroll
-            for (int tr = 0; tr < R; ++tr) run[tr] += rec[q].v[tr] * xv;
+            for (int tr = 0; tr < R; ++tr)
+#pragma unroll
+                for (int e = 0; e < CPL; ++e) run[tr][e] += rec[q].v[tr] * xv[e];
         }
         // groups end their runs with the batch (each batch splits anew);
         // one group keeps its block-row open across batches
@@ -587,10 +652,17 @@ SpmmPlans& spmm_plans() {
     return p;
 }
 
-template <class V, class X, class ACC, int VS, int MB, int KC, bool WIDE>
+// Groups for ntiles tiles times the column chunks, a multiple of 8 (the
+// caller has checked that it fits 32 bits).
+unsigned spmm_grid(int64_t ntiles, int k, int kt) {
+    const uint64_t items = (uint64_t)ntiles * (uint64_t)((k + kt - 1) / kt);
+    return (unsigned)((items + 7) / 8 * 8);
+}
+
+template <class V, class X, class ACC, int VS, int MB, int KC, bool WIDE, int CPL>
 int launch_spmm_cols(const spc5_hip_mat& m, const Plan& p, int maxrows, const void* x, int64_t ldx, void* y,
                      int64_t ldy, int k, int kt, double alpha, double beta, hipStream_t s) {
-    const dim3 grid((unsigned)p.ntiles, (unsigned)((k + kt - 1) / kt));
+    const dim3 grid(spmm_grid(p.ntiles, k, kt));
     const int* tb = p.tile_br->as<int>();
     const int* tv = p.tile_val->as<int>();
     const V* val = reinterpret_cast<const V*>(m.values);
@@ -598,8 +670,9 @@ int launch_spmm_cols(const spc5_hip_mat& m, const Plan& p, int maxrows, const vo
     X* yy = reinterpret_cast<X*>(y);
 #define SPC5_SPMM_COLS(R)                                                                                          \
     {                                                                                                              \
-        const size_t shm = spmm_col_smem_bytes<ACC, R>(maxrows, kt, spmm_col_direct<KC>());                       \
-        hipLaunchKernelGGL((k_spmm_cols<V, X, ACC, R, VS, MB, KC, WIDE>), grid, dim3(64), shm, s, m.bptr, m.hdr, val, \
+        const size_t shm = spmm_col_smem_bytes<ACC, R>(maxrows, kt, spmm_col_direct<KC, CPL>());                  \
+        hipLaunchKernelGGL((k_spmm_cols<V, X, ACC, R, VS, MB, KC, WIDE, CPL>), grid, dim3(64), shm, s, m.bptr, m.hdr, \
+                           val,                                                                                    \
                            xx, ldx, yy, ldy, k, kt, tb, tv, p.ntiles, m.nrows, maxrows, (ACC)alpha, (ACC)beta);       \
     }
     switch (m.r) {
@@ -615,10 +688,17 @@ int launch_spmm_cols(const spc5_hip_mat& m, const Plan& p, int maxrows, const vo
 
 template <class V, class X, class ACC, int VS, int MB, int KC, bool WIDE>
 int launch_spmm_kc(const spc5_hip_mat& m, const Plan& p, int maxrows, const void* x, int64_t ldx, void* y,
-                   int64_t ldy, int k, int kt, double alpha, double beta, hipStream_t s) {
-    if (m.spmm_cols && m.r > 1)
-        return launch_spmm_cols<V, X, ACC, VS, MB, KC, WIDE>(m, p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s);
-    const dim3 grid((unsigned)p.ntiles, (unsigned)((k + kt - 1) / kt));
+                   int64_t ldy, int k, int kt, double alpha, double beta, bool vec, hipStream_t s) {
+    if (m.spmm_cols && m.r > 1) {
+        constexpr int CPL = 16 / (int)sizeof(X);
+        if constexpr (KC >= CPL) {
+            if (vec)
+                return launch_spmm_cols<V, X, ACC, VS, MB, KC, WIDE, CPL>(m, p, maxrows, x, ldx, y, ldy, k, kt, alpha,
+                                                                          beta, s);
+        }
+        return launch_spmm_cols<V, X, ACC, VS, MB, KC, WIDE, 1>(m, p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s);
+    }
+    const dim3 grid(spmm_grid(p.ntiles, k, kt));
     const size_t shm = spmm_smem_bytes<ACC, KC, X>(maxrows, kt, m.r);
     const int* tb = p.tile_br->as<int>();
     const int* tv = p.tile_val->as<int>();
@@ -644,8 +724,22 @@ int launch_spmm_t(const spc5_hip_mat& m, const void* x, int64_t ldx, void* y, in
                   double alpha, double beta, hipStream_t s) {
     int kc = 1;  // lanes per record: the smallest power of two holding the chunk's kt columns
     while (kc < kt) kc <<= 1;
-    // column records with 64-column chunks write Y directly: whole SpMV-sized tiles
-    const bool direct = m.spmm_cols && m.r > 1 && kc == 64;
+    // With SPMM_X16, column records take 16 / sizeof(X) adjacent columns
+    // per lane (one 16-byte X load) when every chunk's width is a multiple
+    // of that and every X row and chunk start is 16-byte aligned.  A
+    // record's X request then takes a half (double) or a quarter (float) of
+    // the lanes, and the address path and the LDS record reads cost that
+    // much less per column: k = 64 in double on the 27-point stencil
+    // (160^3, beta(4,8)) went from 3.65 to 2.95 ms.  For float X it wins or
+    // loses by r and chunk (2.37 -> 2.74 ms at beta(4,8) with 64-column
+    // chunks, 2.28 ms at beta(2,8)), so it is off by default there.
+    const int cpl = 16 / (int)sizeof(X);
+    const bool x16 = m.spmm_x16 < 0 ? sizeof(X) == 8 : m.spmm_x16 == 1;
+    const bool vec = m.spmm_cols && m.r > 1 && x16 && kc >= cpl && k % cpl == 0 && kt % cpl == 0 &&
+                     ldx % cpl == 0 && ((uintptr_t)x & 15u) == 0;
+    // column records with 64-column chunks of one column per lane write Y
+    // directly: whole SpMV-sized tiles
+    const bool direct = m.spmm_cols && m.r > 1 && kc == 64 && !vec;
     const int maxrows = direct ? kMaxRows : spmm_max_rows<ACC>(kt);
     Plan* p = nullptr;
     {
@@ -665,14 +759,19 @@ int launch_spmm_t(const spc5_hip_mat& m, const void* x, int64_t ldx, void* y, in
         p = &it->second;
     }
     if (p->ntiles == 0) return SPC5_HIP_OK;
+    if ((uint64_t)p->ntiles * (uint64_t)((k + kt - 1) / kt) > 0xfffffff8ull) {
+        set_error("spmm: tiles times column chunks exceed 2^32 - 8 workgroups");
+        return SPC5_HIP_ERR_INVALID;
+    }
     // 32-bit X offsets when every X row offset plus a column fits, and a
     // row is under 16 MiB (the 24-bit multiply of the expansion)
     const uint64_t row_bytes = (uint64_t)ldx * sizeof(X);
     const bool wide = row_bytes >= (1ull << 24) || (uint64_t)m.ncols * row_bytes > 0xffffffffull;
 #define SPC5_SPMM_KC(KC)                                                                                         \
     return wide ? launch_spmm_kc<V, X, ACC, VS, MB, KC, true>(m, *p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, \
-                                                               s)                                                 \
-                : launch_spmm_kc<V, X, ACC, VS, MB, KC, false>(m, *p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s)
+                                                               vec, s)                                            \
+                : launch_spmm_kc<V, X, ACC, VS, MB, KC, false>(m, *p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta,  \
+                                                                vec, s)
     switch (kc) {
     case 64: SPC5_SPMM_KC(64);
     case 32: SPC5_SPMM_KC(32);

@@ -497,6 +497,10 @@ int spc5_hip_mat_set_option(spc5_hip_mat_t h, int option, int64_t value) {
         SPC5_CHECK_ARG(value == 0 || value == 1, "SPMM_COLUMNS must be 0 or 1");
         h->spmm_cols = (int)value;
         return SPC5_HIP_OK;
+    case SPC5_HIP_OPT_SPMM_X16:
+        SPC5_CHECK_ARG(value == 0 || value == 1, "SPMM_X16 must be 0 or 1");
+        h->spmm_x16 = (int)value;
+        return SPC5_HIP_OK;
     case SPC5_HIP_OPT_TILE_VALUES: {
         SPC5_CHECK_ARG(value == 0 || (value >= 64 && value <= (1 << 24)), "TILE_VALUES must be 0 or in [64, 2^24]");
         const int target = value == 0 ? default_tile_values(h->nnz) : (int)value;

@@ -13,7 +13,7 @@ import numpy as np
 import torch
 
 from ._lib import (SPC5_HIP_BF16, SPC5_HIP_F32, SPC5_HIP_F64, SPC5_HIP_OPT_ROW_SORT, SPC5_HIP_OPT_SPMM_COLUMNS,
-                   SPC5_HIP_OPT_TILE_VALUES, Spc5Error, check, lib)
+                   SPC5_HIP_OPT_SPMM_X16, SPC5_HIP_OPT_TILE_VALUES, Spc5Error, check, lib)
 
 _VT = {torch.float64: SPC5_HIP_F64, torch.float32: SPC5_HIP_F32, torch.bfloat16: SPC5_HIP_BF16}
 _VEC_DTYPE = {SPC5_HIP_F64: torch.float64, SPC5_HIP_F32: torch.float32, SPC5_HIP_BF16: torch.float32}
@@ -195,6 +195,18 @@ class DeviceMatrix:
     def spmm_columns(self, on: bool):
         check(lib().spc5_hip_mat_set_option(self._h, SPC5_HIP_OPT_SPMM_COLUMNS, 1 if on else 0), "set_option")
         self._spmm_columns = bool(on)
+
+    @property
+    def spmm_x16(self) -> bool:
+        """Column records request 16 bytes of an X row per lane (two doubles
+        or four floats) where X's layout allows it; on by default for double
+        vectors (see SPC5_HIP_OPT_SPMM_X16 in include/spc5_hip.h)."""
+        return getattr(self, "_spmm_x16", self.vec_dtype == torch.float64)
+
+    @spmm_x16.setter
+    def spmm_x16(self, on: bool):
+        check(lib().spc5_hip_mat_set_option(self._h, SPC5_HIP_OPT_SPMM_X16, 1 if on else 0), "set_option")
+        self._spmm_x16 = bool(on)
 
     @property
     def tile_values(self) -> int:

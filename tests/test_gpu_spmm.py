@@ -156,9 +156,41 @@ def test_spmm_strided_and_beta0():
     assert np.abs(Y.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max() * 10
 
 
-def test_spmm_wide_x_rows():
+@pytest.mark.parametrize("dtype,off,width", [(np.float64, 4, 24), (np.float64, 1, 24), (np.float64, 2, 25),
+                                               (np.float64, 0, 8), (np.float32, 4, 24), (np.float32, 2, 24),
+                                               (np.float32, 4, 26), (np.float32, 0, 16)])
+def test_spmm_column_records_vector_x_loads(dtype, off, width):
+    """Column records take 16 / sizeof(X) adjacent columns per lane (one
+    16-byte X load) only when every X row and every chunk start is 16-byte
+    aligned and every chunk's width is a multiple of that count; views that
+    break one of these (an 8-byte offset, a leading dimension or a k that is
+    not a multiple) take one column per lane.  Every case against the dense
+    product of the values the matrix holds."""
+    rp, col, val = clustered_csr(300, 300, seed=3)
+    A, vt, vheld = _matrix_of_kind("f64" if dtype == np.float64 else "f32", rp, col, val, 4, 300)
+    A.spmm_columns = True
+    assert A.spmm_x16 == (dtype == np.float64)  # the default
+    A.spmm_x16 = True
+    rng = np.random.default_rng(2)
+    Xbig = dev(rng.uniform(-1, 1, (300, width)).astype(vt))
+    Ad = dense_of(rp, col, vheld, 300)
+    tol = 1e-12 if vt == np.float64 else 1e-5
+    for k, kt in ((8, 0), (8, 4), (12, 4), (6, 4), (7, 0), (2, 2), (4, 0)):
+        if off + k > width:
+            continue
+        X = Xbig[:, off:off + k]
+        Y = torch.full((300, k), float("nan"), dtype=X.dtype, device="cuda")
+        A.spmm(X, Y, alpha=2.0, beta=0.0, kt=kt)
+        ref = 2.0 * Ad @ X.cpu().numpy().astype(np.float64)
+        err = np.abs(Y.cpu().numpy().astype(np.float64) - ref).max()
+        assert err <= tol * np.abs(ref).max() * 10, (off, width, k, kt, err)
+
+
+@pytest.mark.parametrize("columns", (False, True))
+def test_spmm_wide_x_rows(columns):
     """X with a leading dimension that puts it beyond 4 GiB: the SpMM body
-    with 64-bit X addressing; reference from the CSR."""
+    with 64-bit X addressing, with per-value and with column records (the
+    latter with two columns per lane); reference from the CSR."""
     import numpy as np
     import torch
     from spc5_amd.device import DeviceMatrix
@@ -177,6 +209,7 @@ def test_spmm_wide_x_rows():
     xmap = {int(c): xu[j] for j, c in enumerate(used)}
     A = DeviceMatrix.from_csr(torch.from_numpy(rowptr).cuda(), torch.from_numpy(col).cuda(),
                               torch.from_numpy(v).cuda(), 2, ncols=ncols)
+    A.spmm_columns = columns
     Y = torch.zeros((n, k), dtype=torch.float64, device="cuda")
     A.spmm(X[:, :k], Y, beta=0.0)  # a row-major view with leading dimension ldx
     torch.cuda.synchronize()
@@ -187,20 +220,23 @@ def test_spmm_wide_x_rows():
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("x16", (False, True))
 @pytest.mark.parametrize("kind", ("f64", "f32", "lowp_f32", "lowp_bf16"))
 @pytest.mark.parametrize("r", (2, 4, 8))
-@pytest.mark.parametrize("k,kt", [(2, 0), (3, 0), (16, 0), (16, 4), (64, 0), (70, 0), (70, 32)])
-def test_spmm_column_records(kind, r, k, kt):
+@pytest.mark.parametrize("k,kt", [(2, 0), (3, 0), (16, 0), (16, 4), (64, 0), (64, 32), (70, 0), (70, 32)])
+def test_spmm_column_records(kind, r, k, kt, x16):
     """The column-record schedule (SPC5_HIP_OPT_SPMM_COLUMNS: one X row
-    request per set column of a block, r running row sums) against the
-    dense product of the values the matrix holds, and against the
-    per-value schedule (agreement to rounding)."""
+    request per set column of a block, r running row sums; with x16, 16
+    bytes of the row per lane where the shape allows) against the dense
+    product of the values the matrix holds, and against the per-value
+    schedule (agreement to rounding)."""
     rp, col, val = clustered_csr(577, 430, seed=5 * r + k, empty_frac=0.1)
     A, vt, vheld = _matrix_of_kind(kind, rp, col, val, r, 430)
     rng = np.random.default_rng(k + 7 * r)
     X = rng.uniform(-1, 1, (430, k)).astype(vt)
     Y0 = rng.uniform(-1, 1, (577, k)).astype(vt)
     A.spmm_columns = True
+    A.spmm_x16 = x16
     Yc = dev(Y0.copy())
     A.spmm(dev(X), Yc, alpha=0.75, beta=1.25, kt=kt)
     A.spmm_columns = False
@@ -231,10 +267,11 @@ def test_spmm_column_records_stencil_tuned(kind):
     for r in (2, 4, 8):
         m = DeviceMatrix.from_csr(*d, r)
         mats[r] = m if kind == "f64" else lowp(m, "f32")
-    (r, kt, cols), times = autotune.tune_spmm(mats, 64, reps=2)
-    assert (r, kt, cols) in times and len(times) >= 6
+    (r, kt, cols, x16), times = autotune.tune_spmm(mats, 64, reps=2)
+    assert (r, kt, cols, x16) in times and len(times) >= 9
+    assert {(c, v) for (_, _, c, v) in times} == {(False, False), (True, False), (True, True)}
     A = mats[r]
-    assert A.spmm_columns == cols
+    assert A.spmm_columns == cols and A.spmm_x16 == x16
     vt = np.float64 if kind == "f64" else np.float32
     vheld = val if kind == "f64" else _rounded(val, "lowp_f32")
     rng = np.random.default_rng(64)
@@ -243,4 +280,4 @@ def test_spmm_column_records_stencil_tuned(kind):
     A.spmm(dev(X), Y, alpha=1.0, beta=0.0, kt=kt)
     ref = sp.csr_matrix((vheld, col, rp), shape=(n, n)) @ X.astype(np.float64)
     err = np.abs(Y.cpu().numpy().astype(np.float64) - ref).max() / 54.001
-    assert err <= (1e-12 if kind == "f64" else 1e-5), (r, kt, cols, err)
+    assert err <= (1e-12 if kind == "f64" else 1e-5), (r, kt, cols, x16, err)
