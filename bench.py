@@ -425,10 +425,11 @@ def spmm_benchmark(n=160, ks=(1, 4, 16, 64), steps=10, warmup=2, rs=(2, 4, 8)):
             t = _events_time(lambda: A.spmm(X, Y, kt=kt), steps, warmup)
             a_bytes = A.algorithmic_bytes() - (A.ncols + 2 * A.nrows) * A.vec_dtype.itemsize
             B = a_bytes + k * (A.ncols + 2 * A.nrows) * A.vec_dtype.itemsize
-            kinds = {(False, False): "values", (True, False): "columns", (True, True): "columns x16"}
+            kinds = {(False, False): "values", (False, True): "values x16", (True, False): "columns",
+                     (True, True): "columns x16"}
             case = {"dtype": tag, "k": k, "r": r, "kt": kt,
-                    "records": ("per block column, 16-byte X requests" if x16 else "per block column") if cols
-                    else "per value",
+                    "records": ("per block column" if cols else "per value") +
+                    (", 16-byte X requests" if x16 else ""),
                     "ms": round(t * 1e3, 4), "gflops": round(2.0 * nnz * k / t / 1e9, 1),
                     "gbs": round(B / t / 1e9, 1), "frac_of_peak": round(B / t / 1e9 / HBM_PEAK_GBS, 4), "bytes": B,
                     "candidates_ms": {f"r={cr} kt={ckt} {kinds[(cc, cx)]}": round(v * 1e3, 4)

@@ -158,8 +158,8 @@ int spc5_hip_mat_load(const char* path, spc5_hip_mat_t* out);
  * batch created afterwards uses the new tiles.  No counterpart in the
  * reference (the work split of the GPU kernel). */
 #define SPC5_HIP_OPT_TILE_VALUES 3
-/* SPMM_X16 (0 or 1; default 1 for double vectors, 0 for float): with
- * SPMM_COLUMNS, each lane requests 16 bytes of an X row (two doubles or four
+/* SPMM_X16 (0 or 1; default 1 for double vectors, 0 for float): in
+ * spc5_hip_spmm each lane requests 16 bytes of an X row (two doubles or four
  * floats, adjacent columns) instead of one element, where X allows it
  * (16-byte aligned X, a leading dimension, k and column chunk that are
  * multiples of the element count); otherwise one element per lane.  Fewer

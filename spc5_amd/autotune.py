@@ -223,14 +223,14 @@ def tune_spmm(mats, k: int, kt_candidates=None, reps=5):
     block shapes of one matrix: `mats` maps r -> DeviceMatrix (the same
     matrix converted with that r).  Tries every r, column chunk kt and
     record kind: one record per stored value, or (r > 1) one per set column
-    of a block (SPC5_HIP_OPT_SPMM_COLUMNS), whose X requests are one element
-    or 16 bytes per lane (SPC5_HIP_OPT_SPMM_X16).  Leaves each matrix set to
+    of a block (SPC5_HIP_OPT_SPMM_COLUMNS), and X requests of one element or
+    16 bytes per lane (SPC5_HIP_OPT_SPMM_X16).  Leaves each matrix set to
     its best record kind.  Returns ((r, kt, columns, x16),
-    {(r, kt, columns, x16): seconds}); x16 is False for per-value records."""
+    {(r, kt, columns, x16): seconds})."""
     times = {}
     for r, m in mats.items():
         best_kind, best_t = (False, False), None
-        kinds = ((False, False), (True, False), (True, True)) if r > 1 else ((False, False),)
+        kinds = ((False, False), (False, True)) + (((True, False), (True, True)) if r > 1 else ())
         for cols, x16 in kinds:
             m.spmm_columns = cols
             m.spmm_x16 = x16

@@ -63,9 +63,9 @@ using SpmmFloatAcc = std::conditional<SPC5_SPMM_FLOAT_ACC_DOUBLE != 0, double, f
 // chunks measured slower with it).  Double X keeps vector records: the
 // scalar form measured 2 % slower there (its 512-byte X rows, not
 // occupancy, set the pace).
-template <int KC, class X>
+template <int KC, class X, int CPL = 1>
 constexpr int spmm_cap() {
-    return KC == 64 && sizeof(X) < 8 ? SPC5_SPMM_CAP64_NARROW : SPC5_SPMM_CAP;
+    return KC == 64 && CPL == 1 && sizeof(X) < 8 ? SPC5_SPMM_CAP64_NARROW : SPC5_SPMM_CAP;
 }
 // a batch must hold the largest block (r = 8, VS = 16: 128 values)
 static_assert(SPC5_SPMM_CAP >= 128 && SPC5_SPMM_CAP % 64 == 0, "SPC5_SPMM_CAP: a multiple of 64, at least 128");
@@ -101,9 +101,9 @@ int spmm_max_rows(int kt) {
     return rows;
 }
 
-template <class ACC, int KC, class X>
+template <class ACC, int KC, class X, int CPL>
 size_t spmm_smem_bytes(int maxrows, int kt, int r) {
-    return sizeof(SpmmRec<ACC>) * spmm_cap<KC, X>() + sizeof(ACC) * (size_t)maxrows * kt +
+    return sizeof(SpmmRec<ACC>) * spmm_cap<KC, X, CPL>() + sizeof(ACC) * (size_t)maxrows * kt +
            sizeof(int) * (maxrows / r + 1);
 }
 
@@ -114,6 +114,27 @@ __device__ __forceinline__ X spmm_x(const X* __restrict__ xcol, const X* __restr
         return xcol[(int64_t)col * ldx];
     } else {
         return *reinterpret_cast<const X*>(reinterpret_cast<const char*>(x) + ((unsigned)col + lane_off));
+    }
+}
+
+// CPL adjacent X columns of one record: one X value, or (CPL = 16 /
+// sizeof(X)) one 16-byte load of two doubles or four floats; the caller has
+// checked that every X row start and every lane's first column are then
+// 16-byte aligned.
+template <int CPL, bool WIDE, class X, class ACC>
+__device__ __forceinline__ void spmm_xs(const X* __restrict__ xcol, const X* __restrict__ x, int col, int64_t ldx,
+                                        unsigned lane_off, ACC* out) {
+    if constexpr (CPL == 1) {
+        out[0] = (ACC)spmm_x<WIDE>(xcol, x, col, ldx, lane_off);
+    } else {
+        static_assert(CPL * sizeof(X) == 16, "one 16-byte load per lane");
+        const X* p = WIDE ? xcol + (int64_t)col * ldx
+                          : reinterpret_cast<const X*>(reinterpret_cast<const char*>(x) + ((unsigned)col + lane_off));
+        const uint4 w = *reinterpret_cast<const uint4*>(p);
+        X v[CPL];
+        __builtin_memcpy(v, &w, 16);
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) out[e] = (ACC)v[e];
     }
 }
 
@@ -154,18 +175,22 @@ __device__ __forceinline__ bool spmm_item(int64_t ntiles, int k, int kt, int64_t
 // WIDE: X spans 4 GiB or more (or rows of 16 MiB or more), so records hold
 // columns and X is addressed with 64-bit arithmetic; otherwise they hold
 // 32-bit byte offsets of the X rows, computed with 24-bit multiplies.
-template <class V, class X, class ACC, int R, int VS, int MB, int KC, bool WIDE>
+// CPL: adjacent X columns per lane (1, or 16 / sizeof(X): one 16-byte X
+// request), so KC / CPL lanes share a record.
+template <class V, class X, class ACC, int R, int VS, int MB, int KC, bool WIDE, int CPL>
 __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr, const unsigned char* __restrict__ hdr,
                                                     const V* __restrict__ val, const X* __restrict__ x, int64_t ldx,
                                                     X* __restrict__ y, int64_t ldy, int k, int kt,
                                                     const int* __restrict__ tile_br, const int* __restrict__ tile_val,
                                                     int64_t ntiles, int64_t nrows, int maxrows, ACC alpha, ACC beta) {
-    constexpr int CAP = spmm_cap<KC, X>();
+    constexpr int CAP = spmm_cap<KC, X, CPL>();
     constexpr int K = CAP / 64;
     constexpr int NW = MaskWords<R * MB>::kWords;
-    constexpr int G = 64 / KC;  // record groups per wave
-    // records whose X rows are requested together
-    constexpr int U = KC == 64 ? SPC5_SPMM_U64 : SPC5_SPMM_U;
+    constexpr int LPR = KC / CPL;  // lanes per record
+    constexpr int G = 64 / LPR;    // record groups per wave
+    // records whose X rows are requested together (half as many with
+    // several columns per lane and more than two groups per wave)
+    constexpr int U = (LPR == 64 ? SPC5_SPMM_U64 : SPC5_SPMM_U) / (CPL > 1 && G > 2 ? 2 : 1);
     using Rec = SpmmRec<ACC>;
     extern __shared__ __align__(16) unsigned char smem[];
     Rec* rec = reinterpret_cast<Rec*>(smem);                            // [CAP]
@@ -177,9 +202,9 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
     if (!spmm_item(ntiles, k, kt, t, c0)) return;
     const int ktc = (k - c0) < kt ? (k - c0) : kt;
     const int lane = lane_id();
-    const int grp = lane / KC;
-    const int jl = lane % KC;
-    const bool colok = jl < ktc;
+    const int grp = lane / LPR;
+    const int jl = (lane % LPR) * CPL;  // the lane's first column in the chunk
+    const bool colok = jl < ktc;        // ktc is a multiple of CPL
     const X* __restrict__ xcol = x + c0 + (colok ? jl : 0);
     const unsigned lane_off = (unsigned)(c0 + (colok ? jl : 0)) * (unsigned)sizeof(X);  // byte offset in an X row
     const unsigned row_bytes = (unsigned)ldx * (unsigned)sizeof(X);
@@ -249,9 +274,20 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
         const int per = (total + G - 1) / G;
         int q = grp * per;
         const int qb = q + per < total ? q + per : total;
-        ACC run = (ACC)0;
+        ACC run[CPL];
+#pragma unroll
+        for (int e = 0; e < CPL; ++e) run[e] = (ACC)0;
         int prow = -1;
-        if constexpr (KC == 64 && !WIDE && sizeof(X) < 8) {
+        // the running sums of row prow go to the tile's LDS sums
+        auto close = [&]() {
+            if (prow >= 0 && colok) {
+#pragma unroll
+                for (int e = 0; e < CPL; ++e) atomicAdd(&acc[prow * kt + jl + e], run[e]);
+            }
+#pragma unroll
+            for (int e = 0; e < CPL; ++e) run[e] = (ACC)0;
+        };
+        if constexpr (KC == 64 && CPL == 1 && !WIDE && sizeof(X) < 8) {
             // One group: every record is the same on all lanes.  Lane u reads
             // record q + u (one LDS read for U records) and the fields are
             // broadcast to scalars, so the records take no vector registers
@@ -275,11 +311,10 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     if (ru[u] != prow) {
-                        if (prow >= 0 && colok) atomicAdd(&acc[prow * kt + jl], run);
-                        run = (ACC)0;
+                        close();
                         prow = ru[u];
                     }
-                    run += vu[u] * xv[u];
+                    run[0] += vu[u] * xv[u];
                 }
             }
         } else {
@@ -289,31 +324,32 @@ __global__ __launch_bounds__(64) void k_spmm_tiles(const int* __restrict__ bptr,
                 for (int u = 0; u < U; ++u) rr[u] = rec[q + u];
                 // All U record reads are issued before the first use waits.
                 __builtin_amdgcn_sched_barrier(0);
-                ACC xv[U];
+                ACC xv[U][CPL];
 #pragma unroll
-                for (int u = 0; u < U; ++u) xv[u] = (ACC)spmm_x<WIDE>(xcol, x, rr[u].col, ldx, lane_off);
+                for (int u = 0; u < U; ++u) spmm_xs<CPL, WIDE>(xcol, x, rr[u].col, ldx, lane_off, xv[u]);
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     if (rr[u].row != prow) {
-                        if (prow >= 0 && colok) atomicAdd(&acc[prow * kt + jl], run);
-                        run = (ACC)0;
+                        close();
                         prow = rr[u].row;
                     }
-                    run += rr[u].v * xv[u];
+#pragma unroll
+                    for (int e = 0; e < CPL; ++e) run[e] += rr[u].v * xv[u][e];
                 }
             }
         }
         for (; q < qb; ++q) {
             const Rec r1 = rec[q];
-            const ACC xv = (ACC)spmm_x<WIDE>(xcol, x, r1.col, ldx, lane_off);
+            ACC xv[CPL];
+            spmm_xs<CPL, WIDE>(xcol, x, r1.col, ldx, lane_off, xv);
             if (r1.row != prow) {
-                if (prow >= 0 && colok) atomicAdd(&acc[prow * kt + jl], run);
-                run = (ACC)0;
+                close();
                 prow = r1.row;
             }
-            run += r1.v * xv;
+#pragma unroll
+            for (int e = 0; e < CPL; ++e) run[e] += r1.v * xv[e];
         }
-        if (prow >= 0 && colok) atomicAdd(&acc[prow * kt + jl], run);
+        close();
         v += total;
         b = bn;
         wave_sync();
@@ -358,26 +394,6 @@ constexpr int spmm_col_cap_val() { return R == 2 ? 256 : (R == 4 ? 384 : 512); }
 template <int KC, int CPL>
 constexpr bool spmm_col_direct() { return KC == 64 && CPL == 1; }
 
-// CPL adjacent X columns of one record: one X value, or (CPL = 16 /
-// sizeof(X)) one 16-byte load of two doubles or four floats; the caller has
-// checked that every X row start and every lane's first column are then
-// 16-byte aligned.
-template <int CPL, bool WIDE, class X, class ACC>
-__device__ __forceinline__ void spmm_xs(const X* __restrict__ xcol, const X* __restrict__ x, int col, int64_t ldx,
-                                        unsigned lane_off, ACC* out) {
-    if constexpr (CPL == 1) {
-        out[0] = (ACC)spmm_x<WIDE>(xcol, x, col, ldx, lane_off);
-    } else {
-        static_assert(CPL * sizeof(X) == 16, "one 16-byte load per lane");
-        const X* p = WIDE ? xcol + (int64_t)col * ldx
-                          : reinterpret_cast<const X*>(reinterpret_cast<const char*>(x) + ((unsigned)col + lane_off));
-        const uint4 w = *reinterpret_cast<const uint4*>(p);
-        X v[CPL];
-        __builtin_memcpy(v, &w, 16);
-#pragma unroll
-        for (int e = 0; e < CPL; ++e) out[e] = (ACC)v[e];
-    }
-}
 
 template <class ACC, int R>
 size_t spmm_col_smem_bytes(int maxrows, int kt, bool direct) {
@@ -686,28 +702,19 @@ int launch_spmm_cols(const spc5_hip_mat& m, const Plan& p, int maxrows, const vo
     return SPC5_HIP_OK;
 }
 
-template <class V, class X, class ACC, int VS, int MB, int KC, bool WIDE>
-int launch_spmm_kc(const spc5_hip_mat& m, const Plan& p, int maxrows, const void* x, int64_t ldx, void* y,
-                   int64_t ldy, int k, int kt, double alpha, double beta, bool vec, hipStream_t s) {
-    if (m.spmm_cols && m.r > 1) {
-        constexpr int CPL = 16 / (int)sizeof(X);
-        if constexpr (KC >= CPL) {
-            if (vec)
-                return launch_spmm_cols<V, X, ACC, VS, MB, KC, WIDE, CPL>(m, p, maxrows, x, ldx, y, ldy, k, kt, alpha,
-                                                                          beta, s);
-        }
-        return launch_spmm_cols<V, X, ACC, VS, MB, KC, WIDE, 1>(m, p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s);
-    }
+template <class V, class X, class ACC, int VS, int MB, int KC, bool WIDE, int CPL>
+int launch_spmm_vals(const spc5_hip_mat& m, const Plan& p, int maxrows, const void* x, int64_t ldx, void* y,
+                     int64_t ldy, int k, int kt, double alpha, double beta, hipStream_t s) {
     const dim3 grid(spmm_grid(p.ntiles, k, kt));
-    const size_t shm = spmm_smem_bytes<ACC, KC, X>(maxrows, kt, m.r);
+    const size_t shm = spmm_smem_bytes<ACC, KC, X, CPL>(maxrows, kt, m.r);
     const int* tb = p.tile_br->as<int>();
     const int* tv = p.tile_val->as<int>();
     const V* val = reinterpret_cast<const V*>(m.values);
     const X* xx = reinterpret_cast<const X*>(x);
     X* yy = reinterpret_cast<X*>(y);
 #define SPC5_SPMM_LAUNCH(R)                                                                                   \
-    hipLaunchKernelGGL((k_spmm_tiles<V, X, ACC, R, VS, MB, KC, WIDE>), grid, dim3(64), shm, s, m.bptr, m.hdr, val, \
-                       xx, ldx, yy, ldy, k, kt, tb, tv, p.ntiles, m.nrows, maxrows, (ACC)alpha, (ACC)beta)
+    hipLaunchKernelGGL((k_spmm_tiles<V, X, ACC, R, VS, MB, KC, WIDE, CPL>), grid, dim3(64), shm, s, m.bptr, m.hdr, \
+                       val, xx, ldx, yy, ldy, k, kt, tb, tv, p.ntiles, m.nrows, maxrows, (ACC)alpha, (ACC)beta)
     switch (m.r) {
     case 1: SPC5_SPMM_LAUNCH(1); break;
     case 2: SPC5_SPMM_LAUNCH(2); break;
@@ -719,24 +726,41 @@ int launch_spmm_kc(const spc5_hip_mat& m, const Plan& p, int maxrows, const void
     return SPC5_HIP_OK;
 }
 
+template <class V, class X, class ACC, int VS, int MB, int KC, bool WIDE>
+int launch_spmm_kc(const spc5_hip_mat& m, const Plan& p, int maxrows, const void* x, int64_t ldx, void* y,
+                   int64_t ldy, int k, int kt, double alpha, double beta, bool vec, hipStream_t s) {
+    constexpr int CPL = 16 / (int)sizeof(X);
+    const bool cols = m.spmm_cols && m.r > 1;
+    if constexpr (KC >= CPL) {
+        if (vec)
+            return cols ? launch_spmm_cols<V, X, ACC, VS, MB, KC, WIDE, CPL>(m, p, maxrows, x, ldx, y, ldy, k, kt,
+                                                                            alpha, beta, s)
+                        : launch_spmm_vals<V, X, ACC, VS, MB, KC, WIDE, CPL>(m, p, maxrows, x, ldx, y, ldy, k, kt,
+                                                                            alpha, beta, s);
+    }
+    return cols ? launch_spmm_cols<V, X, ACC, VS, MB, KC, WIDE, 1>(m, p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s)
+                : launch_spmm_vals<V, X, ACC, VS, MB, KC, WIDE, 1>(m, p, maxrows, x, ldx, y, ldy, k, kt, alpha, beta, s);
+}
+
 template <class V, class X, class ACC, int VS, int MB>
 int launch_spmm_t(const spc5_hip_mat& m, const void* x, int64_t ldx, void* y, int64_t ldy, int k, int kt,
                   double alpha, double beta, hipStream_t s) {
     int kc = 1;  // lanes per record: the smallest power of two holding the chunk's kt columns
     while (kc < kt) kc <<= 1;
-    // With SPMM_X16, column records take 16 / sizeof(X) adjacent columns
-    // per lane (one 16-byte X load) when every chunk's width is a multiple
-    // of that and every X row and chunk start is 16-byte aligned.  A
-    // record's X request then takes a half (double) or a quarter (float) of
-    // the lanes, and the address path and the LDS record reads cost that
-    // much less per column: k = 64 in double on the 27-point stencil
-    // (160^3, beta(4,8)) went from 3.65 to 2.95 ms.  For float X it wins or
-    // loses by r and chunk (2.37 -> 2.74 ms at beta(4,8) with 64-column
-    // chunks, 2.28 ms at beta(2,8)), so it is off by default there.
+    // With SPMM_X16, records take 16 / sizeof(X) adjacent columns per lane
+    // (one 16-byte X load) when every chunk's width is a multiple of that
+    // and every X row and chunk start is 16-byte aligned.  A record's X
+    // request then takes a half (double) or a quarter (float) of the lanes,
+    // and the address path and the LDS record reads cost that much less per
+    // column: k = 64 in double on the 27-point stencil (160^3, beta(4,8),
+    // column records) went from 3.65 to 2.95 ms.  For float X it wins or
+    // loses by r and chunk (column records with 64-column chunks: 2.37 ->
+    // 2.74 ms at beta(4,8), 2.28 ms at beta(2,8)), so it is off by default
+    // there.
     const int cpl = 16 / (int)sizeof(X);
     const bool x16 = m.spmm_x16 < 0 ? sizeof(X) == 8 : m.spmm_x16 == 1;
-    const bool vec = m.spmm_cols && m.r > 1 && x16 && kc >= cpl && k % cpl == 0 && kt % cpl == 0 &&
-                     ldx % cpl == 0 && ((uintptr_t)x & 15u) == 0;
+    const bool vec = x16 && kc >= cpl && k % cpl == 0 && kt % cpl == 0 && ldx % cpl == 0 &&
+                     ((uintptr_t)x & 15u) == 0;
     // column records with 64-column chunks of one column per lane write Y
     // directly: whole SpMV-sized tiles
     const bool direct = m.spmm_cols && m.r > 1 && kc == 64 && !vec;

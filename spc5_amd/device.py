@@ -198,9 +198,10 @@ class DeviceMatrix:
 
     @property
     def spmm_x16(self) -> bool:
-        """Column records request 16 bytes of an X row per lane (two doubles
-        or four floats) where X's layout allows it; on by default for double
-        vectors (see SPC5_HIP_OPT_SPMM_X16 in include/spc5_hip.h)."""
+        """Multi-right-hand-side products request 16 bytes of an X row per
+        lane (two doubles or four floats) where X's layout allows it; on by
+        default for double vectors (see SPC5_HIP_OPT_SPMM_X16 in
+        include/spc5_hip.h)."""
         return getattr(self, "_spmm_x16", self.vec_dtype == torch.float64)
 
     @spmm_x16.setter

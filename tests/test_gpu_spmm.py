@@ -71,16 +71,20 @@ def _matrix_of_kind(kind, rp, col, val64, r, ncols):
     return A, np.float32, _rounded(val64, kind)
 
 
+@pytest.mark.parametrize("x16", (False, True))
 @pytest.mark.parametrize("kind", ("f64", "f32", "lowp_f32", "lowp_bf16"))
 @pytest.mark.parametrize("r", (1, 2, 4, 8))
-@pytest.mark.parametrize("k,kt", [(1, 0), (16, 0), (64, 0), (70, 0), (64, 16)])
-def test_spmm_value_kinds(kind, r, k, kt):
-    """Every value kind the library multiplies with k columns: the float and
-    bfloat16 views use the 64-column scalar-record path (k = 64, 70) and the
-    narrower chunks; the product is checked against the dense product of the
-    values the matrix actually holds (normwise 1e-12 double, 1e-5 float)."""
+@pytest.mark.parametrize("k,kt", [(1, 0), (16, 0), (64, 0), (70, 0), (64, 16), (64, 32)])
+def test_spmm_value_kinds(kind, r, k, kt, x16):
+    """Every value kind the library multiplies with k columns, per-value
+    records: the float and bfloat16 views use the 64-column scalar-record
+    path (k = 64, 70) and the narrower chunks; with x16, 16 bytes of an X
+    row per lane where the shape allows.  The product is checked against the
+    dense product of the values the matrix actually holds (normwise 1e-12
+    double, 1e-5 float)."""
     rp, col, val = clustered_csr(601, 450, seed=11 * r + k, empty_frac=0.1)
     A, vt, vheld = _matrix_of_kind(kind, rp, col, val, r, 450)
+    A.spmm_x16 = x16
     rng = np.random.default_rng(k + r)
     X = rng.uniform(-1, 1, (450, k)).astype(vt)
     Y0 = rng.uniform(-1, 1, (601, k)).astype(vt)
@@ -156,19 +160,20 @@ def test_spmm_strided_and_beta0():
     assert np.abs(Y.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max() * 10
 
 
+@pytest.mark.parametrize("columns", (False, True))
 @pytest.mark.parametrize("dtype,off,width", [(np.float64, 4, 24), (np.float64, 1, 24), (np.float64, 2, 25),
                                                (np.float64, 0, 8), (np.float32, 4, 24), (np.float32, 2, 24),
                                                (np.float32, 4, 26), (np.float32, 0, 16)])
-def test_spmm_column_records_vector_x_loads(dtype, off, width):
-    """Column records take 16 / sizeof(X) adjacent columns per lane (one
-    16-byte X load) only when every X row and every chunk start is 16-byte
-    aligned and every chunk's width is a multiple of that count; views that
-    break one of these (an 8-byte offset, a leading dimension or a k that is
-    not a multiple) take one column per lane.  Every case against the dense
-    product of the values the matrix holds."""
+def test_spmm_vector_x_loads(dtype, off, width, columns):
+    """Records of either kind take 16 / sizeof(X) adjacent columns per lane
+    (one 16-byte X load) only when every X row and every chunk start is
+    16-byte aligned and every chunk's width is a multiple of that count;
+    views that break one of these (an 8-byte offset, a leading dimension or
+    a k that is not a multiple) take one column per lane.  Every case
+    against the dense product of the values the matrix holds."""
     rp, col, val = clustered_csr(300, 300, seed=3)
     A, vt, vheld = _matrix_of_kind("f64" if dtype == np.float64 else "f32", rp, col, val, 4, 300)
-    A.spmm_columns = True
+    A.spmm_columns = columns
     assert A.spmm_x16 == (dtype == np.float64)  # the default
     A.spmm_x16 = True
     rng = np.random.default_rng(2)
@@ -268,8 +273,8 @@ def test_spmm_column_records_stencil_tuned(kind):
         m = DeviceMatrix.from_csr(*d, r)
         mats[r] = m if kind == "f64" else lowp(m, "f32")
     (r, kt, cols, x16), times = autotune.tune_spmm(mats, 64, reps=2)
-    assert (r, kt, cols, x16) in times and len(times) >= 9
-    assert {(c, v) for (_, _, c, v) in times} == {(False, False), (True, False), (True, True)}
+    assert (r, kt, cols, x16) in times and len(times) >= 12
+    assert {(c, v) for (_, _, c, v) in times} == {(False, False), (False, True), (True, False), (True, True)}
     A = mats[r]
     assert A.spmm_columns == cols and A.spmm_x16 == x16
     vt = np.float64 if kind == "f64" else np.float32
