@@ -249,6 +249,7 @@ int spc5_hip_batch_create(int nmats, const spc5_hip_mat_t* mats, const void* con
         b->mats.push_back(const_cast<spc5_hip_mat*>(m));
     }
     b->ntiles = (int64_t)tmat.size();
+    SPC5_CHECK_ARG(b->ntiles <= 0x7ffffff0, "batch: more than 2^31 - 16 tiles");
     int dev = 0, ncu = 256;
     SPC5_HIP_TRY(hipGetDevice(&dev));
     SPC5_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));

@@ -17,6 +17,42 @@ def dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
+@pytest.mark.parametrize("nmats,nrows,waves", [(1, 50, 0), (3, 200, 1), (5, 3000, 3), (12, 2500, 8), (12, 2500, 17),
+                                                (12, 2500, 1000), (30, 4000, 0)])
+def test_batch_persistent_wave_counts(nmats, nrows, waves):
+    """The persistent waves walk the tile list with a stride of the wave
+    count: a single tile, one wave, fewer waves than tiles, more waves than
+    tiles and the default count all cover every tile exactly once (each y
+    against the oracle after one product)."""
+    from spc5_amd.batch import Batch
+    from spc5_amd.device import DeviceMatrix
+    rng = np.random.default_rng(nmats * 7 + waves)
+    mats, xs, ys, refs = [], [], [], []
+    for i in range(nmats):
+        n = int(rng.integers(1, nrows + 1))
+        rp, col, v = clustered_csr(n, n, seed=100 + i, empty_frac=0.1)
+        m = DeviceMatrix.from_csr(dev(rp), dev(col), dev(v), 2, ncols=n)
+        m.tile_values = 256  # many small tiles
+        x = rng.uniform(-1, 1, n)
+        y0 = rng.uniform(-1, 1, n)
+        mats.append(m)
+        xs.append(dev(x))
+        ys.append(dev(y0))
+        bptr, hdr, vout, nb = O.convert(2, rp, col, v)
+        xp = np.zeros(n + 8)
+        xp[:n] = x
+        yr = np.zeros(n + 8)
+        if nb:
+            O.beta_spmv(2, n, bptr, hdr, vout, xp, yr)
+        refs.append((rp, v, x, y0, yr[:n]))
+    b = Batch(mats, xs, ys, waves=waves)
+    b.spmv(graph=False)
+    torch.cuda.synchronize()
+    for (rp, v, x, y0, yr), y in zip(refs, ys):
+        assert normwise_error(y.cpu().numpy(), y0 + yr, rp, v, x, y0) <= 1e-12, (b.info(), len(rp) - 1)
+    b.close()
+
+
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 1e-5)])
 @pytest.mark.parametrize("same_r", [0, 1, 2, 4, 8])
 def test_batch_uneven_matrices(dtype, tol, same_r):
