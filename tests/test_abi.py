@@ -64,6 +64,24 @@ def test_python_binding_covers_header():
     assert set(declared_functions("spc5_hip.h")) == set(SIGNATURES)
 
 
+def test_python_constants_match_header():
+    """Every SPC5_HIP_* constant the header defines has the same value in
+    the Python binding (value types, status codes, kernel options)."""
+    from spc5_amd import _lib
+    text = open(os.path.join(INCLUDE, "spc5_hip.h")).read()
+    defs = dict(re.findall(r"^#define\s+(SPC5_HIP_[A-Z0-9_]+)\s+(-?\d+)\b", text, flags=re.M))
+    assert {"SPC5_HIP_OPT_ROW_SORT", "SPC5_HIP_OPT_SPMM_COLUMNS", "SPC5_HIP_OPT_TILE_VALUES",
+            "SPC5_HIP_OPT_SPMM_X16"} <= set(defs)
+    assert len(set(v for k, v in defs.items() if k.startswith("SPC5_HIP_OPT_"))) == \
+        len([k for k in defs if k.startswith("SPC5_HIP_OPT_")]), "option numbers must differ"
+    for name, value in defs.items():
+        if hasattr(_lib, name):
+            assert getattr(_lib, name) == int(value), name
+    for name in ("SPC5_HIP_OPT_ROW_SORT", "SPC5_HIP_OPT_SPMM_COLUMNS", "SPC5_HIP_OPT_TILE_VALUES",
+                 "SPC5_HIP_OPT_SPMM_X16", "SPC5_HIP_F64", "SPC5_HIP_F32", "SPC5_HIP_BF16", "SPC5_HIP_OK"):
+        assert hasattr(_lib, name), name
+
+
 def test_library_loads_and_reports_version():
     import spc5_amd
     assert spc5_amd.lib().spc5_hip_version().startswith(b"spc5_hip")
