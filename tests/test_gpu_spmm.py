@@ -286,3 +286,37 @@ def test_spmm_column_records_stencil_tuned(kind):
     ref = sp.csr_matrix((vheld, col, rp), shape=(n, n)) @ X.astype(np.float64)
     err = np.abs(Y.cpu().numpy().astype(np.float64) - ref).max() / 54.001
     assert err <= (1e-12 if kind == "f64" else 1e-5), (r, kt, cols, x16, err)
+
+
+@pytest.mark.parametrize("kind,r,cols,x16,kt", [("f64", 4, True, True, 64), ("f64", 2, True, True, 16),
+                                                ("f64", 4, True, False, 32), ("lowp_f32", 2, True, True, 64),
+                                                ("lowp_f32", 4, True, False, 64), ("lowp_f32", 2, False, True, 16)])
+def test_spmm_stencil160_columns_equal_spmv(kind, r, cols, x16, kt):
+    """The multi-right-hand-side workload at full size (27-point stencil
+    160^3, k = 64) in the schedules the autotuner picks there: every sampled
+    column of Y = A X equals the SpMV product of that column of X (the SpMV
+    path is pinned against the oracle separately).  Normwise 1e-12 in double,
+    1e-5 in float (the SpMM float path sums in double, the SpMV in float)."""
+    from spc5_amd import synth
+    from spc5_amd.device import DeviceMatrix
+    from spc5_amd.solver import lowp
+    rp, col, val = synth.stencil27(160)
+    n = len(rp) - 1
+    A64 = DeviceMatrix.from_csr(dev(rp), dev(col), dev(val), r)
+    A = A64 if kind == "f64" else lowp(A64, "f32")
+    A.spmm_columns, A.spmm_x16 = cols, x16
+    vt = A.vec_dtype
+    g = torch.Generator(device="cuda").manual_seed(160 + r)
+    X = torch.empty((n, 64), dtype=vt, device="cuda").uniform_(-1, 1, generator=g)
+    Y = torch.empty((n, 64), dtype=vt, device="cuda").uniform_(-1, 1, generator=g)
+    Y0 = Y.clone()
+    A.spmm(X, Y, alpha=1.0, beta=1.0, kt=kt)
+    tol = 1e-12 if vt == torch.float64 else 1e-5
+    scale = 52.001  # max row sum of |A| (26 neighbours of -1, diagonal 26.001) times max |x| (1)
+    for j in (0, 1, 17, 31, 32, 63):
+        yj = Y0[:, j].contiguous()
+        A.spmv(X[:, j].contiguous(), yj)
+        err = (Y[:, j].double() - yj.double()).abs().max().item() / scale
+        assert err <= tol, (kind, r, cols, x16, kt, j, err)
+    del X, Y, Y0
+    torch.cuda.empty_cache()
