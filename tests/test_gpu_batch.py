@@ -218,3 +218,40 @@ def test_batch_banded_recipe(kind, r_rule):
             assert normwise_error(y.cpu().numpy(), want, rp, v * (rep + 1), x, y0) <= tol, (rep, r, len(rp) - 1)
         torch.ones(1000, device="cuda").sum().item()
     b.close()
+
+
+@pytest.mark.parametrize("row_sort", (False, True))
+def test_batch_full_size_equals_each_product(row_sort):
+    """The uneven batch at full size (2,000 banded matrices, rows
+    log-uniform in [1e3, 2e5], bandwidth 4..256, fill 0.3: 74 M rows, 6.0 G
+    values, r per matrix from the autotuner's model), one graph launch: every
+    sampled matrix's y is bit-identical to that matrix's own product (same
+    tiles, same arithmetic), which is pinned against the oracle separately."""
+    from spc5_amd import autotune, synth
+    from spc5_amd.batch import Batch
+    from spc5_amd.device import DeviceMatrix
+    params = synth.uneven_batch_params(2000)
+    mats, rows, nnz = [], 0, 0
+    for (n, bw, seed) in params:
+        rp, col, val = synth.banded_gpu(n, bw, 0.3, seed)
+        r, _ = autotune.choose_r(rp, col, val, mode="model")
+        mats.append(DeviceMatrix.from_csr(rp, col, val, r))
+        rows += n
+        nnz += val.numel()
+        del rp, col, val
+    assert len(mats) == 2000 and rows > 70_000_000 and nnz > 5_000_000_000
+    for m in mats:
+        m.row_sort = row_sort
+    g = torch.Generator(device="cuda").manual_seed(9)
+    xs = [torch.empty(m.ncols, dtype=torch.float64, device="cuda").uniform_(-1, 1, generator=g) for m in mats]
+    ys = [torch.empty(m.nrows, dtype=torch.float64, device="cuda").uniform_(-1, 1, generator=g) for m in mats]
+    y0 = {i: ys[i].clone() for i in range(0, 2000, 83)}
+    b = Batch(mats, xs, ys)
+    b.spmv(graph=True)
+    torch.cuda.synchronize()
+    for i, yi in y0.items():
+        mats[i].spmv(xs[i], yi)
+        assert torch.equal(yi, ys[i]), (i, mats[i].r, mats[i].nrows)
+    b.close()
+    del b, xs, ys, mats
+    torch.cuda.empty_cache()
