@@ -493,12 +493,14 @@ def batch_benchmark(count=2000, fill=0.3, steps=10, warmup=2):
         ys = [torch.zeros(m.nrows, dtype=x.dtype, device="cuda") for m, x in zip(mats, xs)]
         # row-sorted batches (used when every matrix shares r = 2 or 4) and
         # plain ones, for every set; the fastest is the case's time
-        times, infos, wave_times, nbytes = {}, {}, {}, {}
+        times, infos, wave_times, nbytes, tiles, tile_times = {}, {}, {}, {}, {}, {}
         for sname, ms in sets.items():
             for srt in (False, True):
                 for m in ms:
                     m.row_sort = srt
-                # the persistent wave count is the autotuner's choice (timed candidates)
+                # tile size and persistent wave count are the autotuner's choices (timed candidates)
+                tiles[(sname, srt)], tile_times[(sname, srt)] = autotune.tune_batch_tiles(
+                    ms, lambda: Batch(ms, xs, ys))
                 waves, wave_times[(sname, srt)] = autotune.tune_batch_waves(lambda w: Batch(ms, xs, ys, waves=w),
                                                                             reps=3)
                 b = Batch(ms, xs, ys, waves=waves)
@@ -519,13 +521,15 @@ def batch_benchmark(count=2000, fill=0.3, steps=10, warmup=2):
         for m in ms:
             r_hist[str(m.r)] = r_hist.get(str(m.r), 0) + 1
             # tile sizes by power-of-two range: "512-1023" holds 512 <= size < 1024
-            lo = 1 << (max(1, m.tile_target()).bit_length() - 1)
+            lo = 1 << (max(1, tiles[best] or m.tile_target()).bit_length() - 1)
             key = f"{lo}-{2 * lo - 1}"
             tiles_hist[key] = tiles_hist.get(key, 0) + 1
         case = {"dtype": tag, "ms": round(t * 1e3, 3), "gflops": round(2.0 * nnz / t / 1e9, 1),
                 "gbs": round(B / t / 1e9, 1), "frac_of_peak": round(B / t / 1e9 / HBM_PEAK_GBS, 4),
                 "bytes": B, "tiles": info["ntiles"], "persistent_waves": info["waves"], "launch": "hipGraph",
                 "waves_candidates_ms": {str(k): round(v * 1e3, 3) for k, v in wave_times[best].items()},
+                "tile_values": tiles[best],
+                "tile_values_candidates_ms": {str(k): round(v * 1e3, 3) for k, v in tile_times[best].items()},
                 "tile_values_histogram": dict(sorted(tiles_hist.items(), key=lambda kv: int(kv[0].split("-")[0]))),
                 "matrices": sname, "r_histogram": r_hist, "row_sort": srt,
                 "ms_by_candidate": {f"{k[0]}{' row-sorted' if k[1] else ''}": round(v * 1e3, 3)

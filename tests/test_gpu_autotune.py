@@ -97,3 +97,34 @@ def test_batch_keeps_its_tiles_and_wave_tuning():
     b.close()
     waves, times = autotune.tune_batch_waves(lambda w: Batch(mats, xs, ys, waves=w), fractions=(0.5, 1.0), reps=2)
     assert waves == min(times, key=times.get)
+
+
+def test_tune_batch_tiles():
+    """tune_batch_tiles times a batch built at every tile size, leaves every
+    matrix set to the fastest, and a batch built then multiplies correctly."""
+    import numpy as np
+    from spc5_amd import autotune, synth
+    from spc5_amd.batch import Batch
+    from spc5_amd.device import DeviceMatrix
+    mats, xs, ys, host = [], [], [], []
+    for i, (n, bw) in enumerate(((30_000, 40), (12_000, 8), (50_000, 100))):
+        rp, col, val = synth.banded_gpu(n, bw, 0.3, seed=20 + i)
+        mats.append(DeviceMatrix.from_csr(rp, col, val, 4))
+        x = torch.linspace(-1, 1, n, dtype=torch.float64, device="cuda")
+        xs.append(x)
+        ys.append(torch.zeros(n, dtype=torch.float64, device="cuda"))
+        host.append(tuple(t.cpu().numpy() for t in (rp, col, val)) + (x.cpu().numpy(),))
+    best, times = autotune.tune_batch_tiles(mats, lambda: Batch(mats, xs, ys), candidates=(0, 1024, 4096), reps=2)
+    assert set(times) == {0, 1024, 4096} and best == min(times, key=times.get)
+    assert all(m.tile_values == best for m in mats)
+    for y in ys:
+        y.zero_()
+    b = Batch(mats, xs, ys)
+    assert b.info()["ntiles"] == sum(m.info()["ntiles"] for m in mats)
+    b.spmv(graph=False)
+    torch.cuda.synchronize()
+    for (rp, col, val, x), y in zip(host, ys):
+        ref = np.zeros(len(rp) - 1)
+        np.add.at(ref, np.repeat(np.arange(len(rp) - 1), np.diff(rp)), val * x[col])
+        assert np.abs(y.cpu().numpy() - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()) * 50
+    b.close()
