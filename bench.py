@@ -453,7 +453,7 @@ def batch_benchmark(count=2000, fill=0.3, steps=10, warmup=2):
     and float (beta(r,8) with float values)."""
     import torch
     from spc5_amd import autotune, synth
-    from spc5_amd.batch import Batch
+    from spc5_amd.batch import ONE_WAVE_PER_TILE, Batch
     from spc5_amd.device import DeviceMatrix
     from spc5_amd.solver import lowp
 
@@ -498,9 +498,10 @@ def batch_benchmark(count=2000, fill=0.3, steps=10, warmup=2):
             for srt in (False, True):
                 for m in ms:
                     m.row_sort = srt
-                # tile size and persistent wave count are the autotuner's choices (timed candidates)
+                # tile size (timed with one wave per tile) and persistent wave
+                # count are the autotuner's choices (timed candidates)
                 tiles[(sname, srt)], tile_times[(sname, srt)] = autotune.tune_batch_tiles(
-                    ms, lambda: Batch(ms, xs, ys))
+                    ms, lambda: Batch(ms, xs, ys, waves=ONE_WAVE_PER_TILE))
                 waves, wave_times[(sname, srt)] = autotune.tune_batch_waves(lambda w: Batch(ms, xs, ys, waves=w),
                                                                             reps=3)
                 b = Batch(ms, xs, ys, waves=waves)

@@ -298,7 +298,9 @@ typedef struct spc5_hip_batch* spc5_hip_batch_t;
 /* Group nmats matrix handles (one value type and VS; each its own r) with
  * their device vectors x_i, y_i.  The tiles of all matrices form one work
  * list walked by `waves` persistent waves (0: as many as the kernel keeps
- * resident, 12-28 per compute unit; a kernel with only that r's code runs
+ * resident, 12-28 per compute unit; more waves than tiles means one wave per
+ * tile, the tiles then dealt out by the hardware's workgroup dispatcher; a
+ * kernel with only that r's code runs
  * when all matrices share one r, with row-sorted batches when that r is 2
  * or 4 and every handle has SPC5_HIP_OPT_ROW_SORT set at creation); the
  * launch is captured once into a hipGraph.  Handles and vectors are

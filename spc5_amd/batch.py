@@ -8,9 +8,15 @@ from ._lib import Spc5Error, check, lib
 from .device import DeviceMatrix, _need_cuda, _ptr, stream_ptr
 
 
+# `waves` for one wave per tile (any count above the tile count is capped there)
+ONE_WAVE_PER_TILE = 2**31 - 1
+
+
 class Batch:
     """y_i += A_i x_i for all i in one launch.  Matrices may use different r
-    but share value type and VS; the x_i / y_i tensors are kept alive here."""
+    but share value type and VS; the x_i / y_i tensors are kept alive here.
+    `waves`: persistent waves walking the tile list (0: the kernel's resident
+    count; ONE_WAVE_PER_TILE: one wave per tile)."""
 
     def __init__(self, mats: list[DeviceMatrix], xs, ys, waves: int = 0):
         if not mats or not (len(mats) == len(xs) == len(ys)):
