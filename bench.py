@@ -684,6 +684,7 @@ def main():
         y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
         for r in [int(s) for s in args.formats.split(",")]:
             m = DeviceMatrix.from_csr(d_rp, d_col, d_val32, r)
+            tile, tile_times = autotune.tune_tiles(m) if not args.no_tune_tiles else (0, {})
             ms, _ = time_spmv(m, x32, y32, args.steps, args.warmup)
             ms_med, ms_min = step_times(m, x32, y32, args.steps)
             B = m.algorithmic_bytes()
@@ -692,9 +693,12 @@ def main():
                                           "ms": round(ms, 5), "ms_median": round(ms_med, 5),
                                           "ms_min": round(ms_min, 5),
                                           "gflops": round(2.0 * nnz / (ms * 1e-3) / 1e9, 2),
-                                          "gbs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
+                                          "gbs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+                                          "tile_values": m.tile_target(),
+                                          "tile_candidates_ms": {str(k): round(v * 1e3, 4)
+                                                                 for k, v in tile_times.items()}}
             log(f"[bench] FP32 beta({r},{m.vs}): {ms:.4f} ms  {gbs:.0f} GB/s  "
-                f"({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)")
+                f"({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s) (tile size {m.tile_target()} values)")
             del m
         del d_val32, x32, y32
         torch.cuda.empty_cache()
