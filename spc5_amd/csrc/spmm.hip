@@ -3,21 +3,23 @@
 // dimension ldx) and Y (nrows x k, ldy).  The batching form; no counterpart
 // in the reference.
 //
-// One wave per (tile, column chunk of kt <= 64 columns).  Per batch of up
-// to 64 blocks the tile's headers are decoded as in the SpMV tile (headers
-// prefetched, values read once, coalesced, from the value stream) and every
-// value becomes a 16-byte LDS record (value, column, tile-local row).  The
-// wave then splits into 64/KC groups of KC lanes (one lane per column of
-// the chunk); each group walks a contiguous run of the batch's records,
-// requests the X rows of U records at a time (one contiguous kt-wide row
-// segment per record; with 64-column chunks of float or bfloat16 X the
-// whole wave is one group and the records are broadcast to scalar
-// registers) and keeps the running row
-// sum in a register, adding it to the tile's LDS row x column sums only when
-// the row changes.  With
-// kt = k the matrix is fetched once for all k columns; smaller kt re-reads
-// the matrix k/kt times with a smaller X working set (the autotuner picks
-// kt).
+// One wave per (tile, column chunk of kt <= 64 columns); the grid lists the
+// pairs chunk by chunk and each XCD walks one contiguous eighth of the list
+// (spmm_item).  Per batch of up to 64 blocks the tile's headers are decoded
+// as in the SpMV tile (headers prefetched, values read once, coalesced, from
+// the value stream) and every value becomes a 16-byte LDS record (value,
+// column, tile-local row), or with column records every set column of a
+// block one record (k_spmm_cols).  The wave then splits into groups of
+// KC / CPL lanes, each lane taking CPL adjacent columns of the chunk (one
+// element, or 16 bytes of the X row with SPC5_HIP_OPT_SPMM_X16); each group
+// walks a contiguous run of the batch's records, requests the X rows of U
+// records at a time (one contiguous kt-wide row segment per record; with
+// 64-column chunks of float or bfloat16 X and one column per lane the whole
+// wave is one group and the records are broadcast to scalar registers) and
+// keeps the running row sums in registers, adding them to the tile's LDS
+// row x column sums only when the row changes.  With kt = k the matrix is
+// fetched once for all k columns; smaller kt re-reads the matrix k/kt times
+// with a smaller X working set (the autotuner picks kt).
 #include "spc5_mat.hpp"
 #include "spmv_tile.hpp"
 
